@@ -1,0 +1,210 @@
+/*
+ * rbfx.h -- C ABI of the MI355X (gfx950) RBF-FD weight engine.
+ *
+ * The library replaces three call sites of RadialBasisFunctions.jl (v0.7.0):
+ *
+ *   rbfx_knn            find_neighbors(data, eval_points, k)         src/utils.jl:12-18
+ *                       (KDTree + knn(..., sortres=true); adjl is 1-based in Julia,
+ *                       0-based here)
+ *   rbfx_build_weights  build_weights_kernel(data, eval_points, adjl, basis,
+ *                       Lrbf, Lmon, mon, boundary_data; batch_size, device)
+ *                                                                    src/solve/execution.jl:128-174
+ *                       (the device gate at execution.jl:140-148 is where a Julia
+ *                       caller routes a non-CPU `device` to this entry point)
+ *   rbfx_spmv           _eval_op(op, x) = op.weights * x  and  mul!(y, op, x, a, b)
+ *                                                                    src/operators/operators.jl:217-218, 529-542
+ *
+ * Conventions
+ *   - Every pointer inside an args struct is a DEVICE pointer (HBM) unless noted.
+ *   - Coordinates are FP64, array-of-structs (point-major), `dim` doubles per point:
+ *     the memory layout of Julia's Vector{SVector{dim,Float64}}.
+ *   - Indices are 0-based int32 (the Julia side subtracts 1).
+ *   - Calls are blocking: the library synchronises its stream before returning,
+ *     as KernelAbstractions.synchronize does at execution.jl:264.
+ *   - Return value 0 = success; otherwise an RBFX_E* code, with a message from
+ *     rbfx_last_error() (thread-local).
+ *   - No exceptions cross this boundary. No allocator types appear in it.
+ */
+#ifndef RBFX_H
+#define RBFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------- */
+enum {
+    RBFX_OK = 0,
+    RBFX_EINVAL = 1,     /* bad argument: maps to Julia ArgumentError           */
+    RBFX_EDIM = 2,       /* size mismatch: maps to Julia DimensionMismatch      */
+    RBFX_EDEVICE = 3,    /* HIP runtime failure                                 */
+    RBFX_ESINGULAR = 4,  /* at least one stencil system was singular: maps to
+                            LinearAlgebra.SingularException from bunchkaufman!  */
+    RBFX_EUNSUPPORTED = 5
+};
+
+/* ---- basis kinds (src/basis/) -------------------------------------- */
+enum {
+    RBFX_PHS1 = 1,       /* phi = r    polyharmonic_spline.jl:142-143 */
+    RBFX_PHS3 = 3,       /* phi = r^3  polyharmonic_spline.jl:227-228 */
+    RBFX_PHS5 = 5,       /* phi = r^5  polyharmonic_spline.jl:313-314 */
+    RBFX_PHS7 = 7,       /* phi = r^7  polyharmonic_spline.jl:399-400 */
+    RBFX_IMQ = 10,       /* phi = 1/sqrt((eps r)^2+1)  inverse_multiquadric.jl:33-34 */
+    RBFX_GAUSSIAN = 11   /* phi = exp(-(eps r)^2)      gaussian.jl:122-123 */
+};
+
+/* ---- operator term kinds ------------------------------------------------
+ * The reference represents an operator L as a Julia callable applied to the
+ * basis (api.jl:30-35). The closed set that its named operators and algebra
+ * produce is lowered to a table of terms; one RHS column is the sum of its terms
+ * (SumKernel/ScaledKernel, operator_algebra.jl:97-114).                        */
+enum {
+    RBFX_OP_VALUE = 0,   /* identity: Regrid()(b)=b regridding.jl:7, Identity operator_algebra.jl:83-84 */
+    RBFX_OP_D1 = 1,      /* d/dx_dim1          partial.jl:53 -> basis d functor          */
+    RBFX_OP_D2 = 2,      /* d^2/dx_dim1^2      partial.jl:53 -> basis d2 functor         */
+    RBFX_OP_MIXED = 3,   /* d^2/dx_dim1 dx_dim2  mixed_partial.jl:393-400, basis.jl:87-92 */
+    RBFX_OP_LAP = 4      /* laplacian          laplacian.jl:7 (PHS constants as in the
+                            reference: 2/r, 12r, 30r^3, 56r^5 in every dimension)          */
+};
+
+/* ---- precision / method ------------------------------------------------- */
+enum {
+    RBFX_PREC_FP64 = 0,          /* plain method: FP64 assembly + pivoted LDL^T      */
+    RBFX_PREC_FP32_REFINED = 1,  /* FP32 factorisation, FP64 residual + refinement,
+                                    FP64 re-solve of stencils that do not converge  */
+    RBFX_PREC_LEARNED = 2        /* MLP surrogate, FP64 residual acceptance,
+                                    FP32_REFINED fallback (needs a loaded model)    */
+};
+
+/* ---- per-stencil status codes (written to args.status) ------------------ */
+enum {
+    RBFX_ST_OK = 0,              /* solved by the requested method                  */
+    RBFX_ST_DIRICHLET = 1,       /* Dirichlet eval node: identity row               */
+    RBFX_ST_REFINED = 2,         /* FP32 factorisation + FP64 refinement accepted   */
+    RBFX_ST_FALLBACK64 = 3,      /* refinement did not converge: FP64 re-solve      */
+    RBFX_ST_LEARNED = 4,         /* surrogate prediction accepted                   */
+    RBFX_ST_SINGULAR = 8,        /* zero pivot: system singular                     */
+    RBFX_ST_EVAL_NOT_FOUND = 9,  /* Hermite stencil without its eval point
+                                    (assembly.jl:35-46 raises ArgumentError)        */
+    RBFX_ST_TOO_LARGE = 10       /* n = k + m beyond the supported maximum          */
+};
+
+#define RBFX_MAX_TERMS 16
+#define RBFX_MAX_COLS 8
+#define RBFX_MAX_N 64            /* largest k + m one wavefront factorises */
+
+typedef struct rbfx_basis {
+    int32_t kind;        /* RBFX_PHS* / RBFX_IMQ / RBFX_GAUSSIAN             */
+    int32_t poly_deg;    /* -1 disables the polynomial block (utils.jl:57-66) */
+    double eps;          /* shape parameter (IMQ, Gaussian)                   */
+} rbfx_basis;
+
+typedef struct rbfx_term {
+    int32_t col;         /* RHS column this term contributes to */
+    int32_t kind;        /* RBFX_OP_*                            */
+    int32_t dim1;        /* 0-based                              */
+    int32_t dim2;        /* 0-based (MIXED only)                 */
+    double coeff;        /* ScaledOperator alpha                 */
+} rbfx_term;
+
+typedef struct rbfx_operator {
+    int32_t n_cols;      /* r: 1 for scalar operators, dim for Jacobian/gradient */
+    int32_t n_terms;
+    rbfx_term terms[RBFX_MAX_TERMS];
+} rbfx_operator;
+
+/* Arguments of rbfx_build_weights (execution.jl:128-174).
+ * Hermite data mirrors BoundaryData (types.jl:178-182) plus the
+ * global_to_boundary map (execution.jl:78-93), 0-based, -1 for interior.     */
+typedef struct rbfx_build_args {
+    int32_t dim;                 /* 1..3                                              */
+    int32_t k;                   /* stencil size; the largest one when adj_ptr != NULL */
+    int64_t n_data;
+    int64_t n_eval;
+    const double* data;          /* n_data*dim                                        */
+    const double* eval;          /* n_eval*dim                                        */
+    const int64_t* adj_ptr;      /* n_eval+1 offsets into adj_idx, or NULL (=i*k)     */
+    const int32_t* adj_idx;      /* stencil members, nearest first                    */
+    rbfx_basis basis;
+    rbfx_operator op;
+    /* Hermite boundary data (all NULL for interior-only builds) */
+    const uint8_t* is_boundary;  /* n_data                                            */
+    const int32_t* bnd_index;    /* n_data: index into bc arrays or -1                */
+    const double* bc_alpha;      /* n_bnd                                             */
+    const double* bc_beta;       /* n_bnd                                             */
+    const double* normals;       /* n_bnd*dim                                         */
+    int64_t n_bnd;               /* number of boundary points                         */
+    /* output CSR (row pointers exact as count_nonzeros, execution.jl:39-69) */
+    const int64_t* out_ptr;      /* n_eval+1, or NULL (=i*k)                          */
+    int32_t* out_col;            /* nnz, or NULL to skip (fixed k: equals adj_idx)    */
+    double* out_val;             /* n_cols * out_nnz, column c at out_val + c*out_nnz */
+    int64_t out_nnz;
+    int32_t* status;             /* n_eval per-stencil status, or NULL                */
+    int32_t precision;           /* RBFX_PREC_*                                       */
+    int32_t refine_steps;        /* FP32_REFINED: refinement updates (0 -> 3)         */
+    double accept_tol;           /* relative residual tolerance (0 -> default)        */
+    void* stream;                /* hipStream_t, NULL = default stream                */
+} rbfx_build_args;
+
+/* Arguments of rbfx_spmv: Y_v = alpha * W_v X + beta * Y_v for every value array v.
+ * W_v share row pointers and column indices (one stencil set, several operators).  */
+typedef struct rbfx_spmv_args {
+    int64_t n_rows;
+    int64_t n_cols;              /* rows of X (data points)                            */
+    const int64_t* row_ptr;      /* n_rows+1, or NULL (=i*k)                           */
+    int32_t k;                   /* fixed row length when row_ptr == NULL              */
+    const int32_t* col_idx;
+    const double* vals;          /* n_valarrays * nnz                                  */
+    int64_t nnz;
+    int32_t n_valarrays;
+    int32_t n_fields;            /* F right-hand fields                                */
+    const double* x;             /* X[p, f] at x[p*x_ld_point + f*x_ld_field]          */
+    int64_t x_ld_point;
+    int64_t x_ld_field;
+    double* y;                   /* Y_v[i, f] at y[v*y_ld_val + i*y_ld_point + f*y_ld_field] */
+    int64_t y_ld_point;
+    int64_t y_ld_field;
+    int64_t y_ld_val;
+    double alpha;
+    double beta;
+    void* stream;
+} rbfx_spmv_args;
+
+/* ---- entry points -------------------------------------------------------- */
+
+/* Select the HIP device for this thread and check it is a gfx950 part. */
+int rbfx_init(int32_t device);
+
+/* Message describing the last failure on this thread ("" if none). */
+const char* rbfx_last_error(void);
+
+/* ABI version: bumped whenever a struct above changes layout. */
+int32_t rbfx_abi_version(void);
+
+/* find_neighbors (utils.jl:12-18): for each query, the k nearest data points by
+ * Euclidean distance, ascending, ties broken by lower index. out_idx is
+ * n_query*k int32. With q_ptr != NULL, query i receives q_ptr[i+1]-q_ptr[i] <= k
+ * neighbours at out_idx + q_ptr[i] (variable-size stencils).                    */
+int rbfx_knn(const double* data, int64_t n_data, const double* query, int64_t n_query,
+             int32_t dim, int32_t k, const int64_t* q_ptr, int32_t* out_idx,
+             void* stream);
+
+/* build_weights_kernel (execution.jl:128-174) and the per-stencil solve it drives
+ * (_build_stencil!, assembly.jl:201-222; Hermite dispatch assembly.jl:240-467).  */
+int rbfx_build_weights(const rbfx_build_args* args);
+
+/* op.weights * X (operators.jl:217-218) and mul!(y, op, x, alpha, beta)
+ * (operators.jl:536-542) for one or several value arrays sharing a pattern.      */
+int rbfx_spmv(const rbfx_spmv_args* args);
+
+/* Number of stencils per status code from the last rbfx_build_weights on this
+ * thread (counts[0..15]); lets the host raise the reference's exception types. */
+int rbfx_last_build_counts(int64_t counts[16]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RBFX_H */

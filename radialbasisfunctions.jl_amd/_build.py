@@ -1,0 +1,64 @@
+"""Build librbfx.so (the gfx950 C-ABI library) in-tree with hipcc.
+
+Every .hip file under csrc/ is compiled to an object in _build/ (in parallel), then
+linked into radialbasisfunctions.jl_amd/librbfx.so.
+"""
+import concurrent.futures as cf
+import os
+import pathlib
+import subprocess
+import sys
+import time
+
+PKG = pathlib.Path(__file__).resolve().parent
+ROOT = PKG.parent
+CSRC = PKG / "csrc"
+OBJ = PKG / "_build"
+LIB = PKG / "librbfx.so"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    # no packed-FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+    "-I", str(ROOT / "include"),
+    "-I", str(CSRC),
+]
+
+
+def _compile(src, verbose):
+    obj = OBJ / (src.stem + ".o")
+    cmd = [HIPCC, "-c", *FLAGS, "-o", str(obj), str(src)]
+    t0 = time.time()
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src.name}:\n{p.stderr[-6000:]}")
+    if verbose:
+        print(f"  compiled {src.name} in {time.time() - t0:.1f}s", flush=True)
+    return obj
+
+
+def build_library(jobs=None, verbose=True):
+    OBJ.mkdir(exist_ok=True)
+    srcs = sorted(CSRC.glob("*.hip"))
+    jobs = jobs or min(len(srcs), os.cpu_count() or 4)
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(s, verbose), srcs))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed:\n{p.stderr[-6000:]}")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"  linked {LIB}", flush=True)
+    return LIB
+
+
+if __name__ == "__main__":
+    build_library(verbose=True)
+    sys.exit(0)

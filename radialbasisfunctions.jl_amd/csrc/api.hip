@@ -1,0 +1,327 @@
+// C ABI of the RBF-FD weight engine (declared in include/rbfx.h).
+//
+// Argument validation follows the reference's error behaviour:
+//   ArgumentError for unsupported normal forms (src/solve/api.jl:382-397),
+//   ArgumentError for a Hermite stencil without its eval point (assembly.jl:35-46),
+//   SingularException from bunchkaufman! for singular stencils (assembly.jl:220-222),
+// reported here as RBFX_EINVAL / RBFX_ESINGULAR with a message.
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "build.h"
+#include "knn.h"
+#include "rbfx.h"
+#include "spmv.h"
+
+namespace {
+
+thread_local std::string g_err;
+thread_local int64_t g_counts[16];
+
+int set_err(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return set_err(RBFX_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
+    } while (0)
+
+// per-device library scratch (grow-only)
+struct DeviceState {
+    rbfx::KnnWorkspace knn;
+    int32_t* fail_list = nullptr;
+    size_t cap_fail = 0;
+    int32_t* fail_count = nullptr;
+    int32_t* status = nullptr;
+    size_t cap_status = 0;
+    unsigned long long* hist = nullptr;
+};
+
+std::mutex g_mu;
+DeviceState g_dev[64];
+
+DeviceState& dev_state()
+{
+    int d = 0;
+    (void)hipGetDevice(&d);
+    return g_dev[d & 63];
+}
+
+hipError_t grow(void** p, size_t& cap, size_t need)
+{
+    if (need <= cap) return hipSuccess;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    cap = 0;
+    hipError_t e = hipMalloc(p, need);
+    if (e == hipSuccess) cap = need;
+    return e;
+}
+
+__global__ void k_hist16(const int32_t* __restrict__ st, int64_t n, unsigned long long* __restrict__ h)
+{
+    __shared__ unsigned int loc[16];
+    if (threadIdx.x < 16) loc[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&loc[st[i] & 15], 1u);
+    __syncthreads();
+    if (threadIdx.x < 16 && loc[threadIdx.x]) atomicAdd(&h[threadIdx.x], (unsigned long long)loc[threadIdx.x]);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+__global__ void k_zero_i32(int32_t* p) { *p = 0; }
+
+int n_mono_host(int dim, int deg)
+{
+    if (deg < 0) return 0;
+    int num = 1, den = 1;
+    for (int i = 1; i <= dim; ++i) { num *= (deg + i); den *= i; }
+    return num / den;
+}
+
+bool is_phs(int k) { return k == RBFX_PHS1 || k == RBFX_PHS3 || k == RBFX_PHS5 || k == RBFX_PHS7; }
+
+}  // namespace
+
+extern "C" {
+
+int32_t rbfx_abi_version(void) { return 1; }
+
+const char* rbfx_last_error(void) { return g_err.c_str(); }
+
+int rbfx_init(int32_t device)
+{
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(RBFX_EINVAL, "device %d out of range (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(RBFX_EDEVICE, "rbfx kernels are built for gfx950; device %d is %s", device, prop.gcnArchName);
+    g_err.clear();
+    return RBFX_OK;
+}
+
+int rbfx_last_build_counts(int64_t counts[16])
+{
+    for (int i = 0; i < 16; ++i) counts[i] = g_counts[i];
+    return RBFX_OK;
+}
+
+int rbfx_knn(const double* data, int64_t n_data, const double* query, int64_t n_query, int32_t dim, int32_t k,
+             const int64_t* q_ptr, int32_t* out_idx, void* stream)
+{
+    if (!data || !query || !out_idx) return set_err(RBFX_EINVAL, "rbfx_knn: null pointer argument");
+    if (dim < 1 || dim > 3) return set_err(RBFX_EINVAL, "rbfx_knn: dim must be 1, 2 or 3 (got %d)", dim);
+    if (k < 1 || k > n_data) return set_err(RBFX_EINVAL, "rbfx_knn: need 1 <= k <= n_data (k=%d, n_data=%lld)", k, (long long)n_data);
+    if (k > 64) return set_err(RBFX_EUNSUPPORTED, "rbfx_knn: k=%d exceeds the supported maximum of 64", k);
+    if (n_query == 0) return RBFX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState& ds = dev_state();
+    HIPCHK(rbfx::run_knn(ds.knn, data, n_data, query, n_query, dim, k, q_ptr, out_idx, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+int rbfx_build_weights(const rbfx_build_args* a)
+{
+    if (!a) return set_err(RBFX_EINVAL, "rbfx_build_weights: null args");
+    for (int i = 0; i < 16; ++i) g_counts[i] = 0;
+    if (a->dim < 1 || a->dim > 3) return set_err(RBFX_EINVAL, "dim must be 1, 2 or 3 (got %d)", a->dim);
+    if (!a->data || !a->eval || !a->adj_idx || !a->out_val)
+        return set_err(RBFX_EINVAL, "rbfx_build_weights: data, eval, adj_idx and out_val are required");
+    const int deg = a->basis.poly_deg;
+    if (deg < -1) return set_err(RBFX_EINVAL, "poly_deg must be >= -1 (got %d)", deg);
+    if (deg > rbfx::kMaxPolyDeg)
+        return set_err(RBFX_EUNSUPPORTED, "poly_deg %d above %d is not built for the device", deg, rbfx::kMaxPolyDeg);
+    const int kind = a->basis.kind;
+    if (!(is_phs(kind) || kind == RBFX_IMQ || kind == RBFX_GAUSSIAN))
+        return set_err(RBFX_EINVAL, "unknown basis kind %d", kind);
+    if ((kind == RBFX_IMQ || kind == RBFX_GAUSSIAN) && !(a->basis.eps > 0))
+        return set_err(RBFX_EINVAL, "Shape parameter eps must be > 0 (got eps=%g)", a->basis.eps);
+    const rbfx_operator& op = a->op;
+    if (op.n_cols < 1 || op.n_cols > RBFX_MAX_COLS) return set_err(RBFX_EINVAL, "n_cols=%d out of range", op.n_cols);
+    if (op.n_terms < 1 || op.n_terms > RBFX_MAX_TERMS) return set_err(RBFX_EINVAL, "n_terms=%d out of range", op.n_terms);
+    for (int t = 0; t < op.n_terms; ++t) {
+        const rbfx_term& tm = op.terms[t];
+        if (tm.col < 0 || tm.col >= op.n_cols) return set_err(RBFX_EINVAL, "term %d: column %d out of range", t, tm.col);
+        if (tm.kind < RBFX_OP_VALUE || tm.kind > RBFX_OP_LAP) return set_err(RBFX_EINVAL, "term %d: unknown kind", t);
+        if ((tm.kind == RBFX_OP_D1 || tm.kind == RBFX_OP_D2 || tm.kind == RBFX_OP_MIXED) &&
+            (tm.dim1 < 0 || tm.dim1 >= a->dim))
+            return set_err(RBFX_EINVAL, "term %d: dimension %d out of range", t, tm.dim1);
+        if (tm.kind == RBFX_OP_MIXED && (tm.dim2 < 0 || tm.dim2 >= a->dim))
+            return set_err(RBFX_EINVAL, "term %d: dimension %d out of range", t, tm.dim2);
+    }
+    const int M = n_mono_host(a->dim, deg);
+    if (a->k < 1) return set_err(RBFX_EINVAL, "stencil size k must be >= 1");
+    if (a->k + M > RBFX_MAX_N)
+        return set_err(RBFX_EUNSUPPORTED, "stencil system k+m=%d exceeds the device maximum %d", a->k + M, RBFX_MAX_N);
+    if (a->n_eval == 0) return RBFX_OK;
+    if (a->precision < RBFX_PREC_FP64 || a->precision > RBFX_PREC_FP32_REFINED)
+        return set_err(RBFX_EUNSUPPORTED, "precision mode %d is not available through rbfx_build_weights", a->precision);
+    hipStream_t st = (hipStream_t)a->stream;
+
+    // Hermite: the normal-form guard of _check_normal_form_support (api.jl:382-397)
+    if (a->is_boundary) {
+        if (!a->bnd_index || (a->n_bnd > 0 && (!a->bc_alpha || !a->bc_beta || !a->normals)))
+            return set_err(RBFX_EINVAL, "Hermite build needs bnd_index, bc_alpha, bc_beta and normals");
+        if (a->n_bnd > 0) {
+            std::vector<double> al(a->n_bnd), be(a->n_bnd);
+            HIPCHK(hipMemcpy(al.data(), a->bc_alpha, sizeof(double) * a->n_bnd, hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(be.data(), a->bc_beta, sizeof(double) * a->n_bnd, hipMemcpyDeviceToHost));
+            bool needs_normal = false;
+            for (int64_t i = 0; i < a->n_bnd; ++i)
+                if (!(al[i] == 1.0 && be[i] == 0.0)) needs_normal = true;
+            bool supported = is_phs(kind);
+            for (int t = 0; t < op.n_terms; ++t) {
+                const int kd = op.terms[t].kind;
+                if (!(kd == RBFX_OP_D1 || kd == RBFX_OP_D2 || kd == RBFX_OP_LAP)) supported = false;
+            }
+            if (needs_normal && !supported)
+                return set_err(RBFX_EINVAL,
+                               "Neumann/Robin boundary conditions require the operator's basis action to support the "
+                               "normal-derivative form Lrbf(x, xi, normal); normal-form methods exist only for PHS "
+                               "bases with the d, grad, d2 and laplacian operators");
+        }
+    }
+
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState& ds = dev_state();
+    int32_t* status = a->status;
+    if (!status) {
+        HIPCHK(grow((void**)&ds.status, ds.cap_status, sizeof(int32_t) * (size_t)a->n_eval));
+        status = ds.status;
+    }
+    if (!ds.hist) HIPCHK(hipMalloc((void**)&ds.hist, 16 * sizeof(unsigned long long)));
+
+    rbfx::BuildDev P;
+    std::memset(&P, 0, sizeof(P));
+    P.n_data = a->n_data;
+    P.n_eval = a->n_eval;
+    P.data = a->data;
+    P.eval = a->eval;
+    P.adj_ptr = a->adj_ptr;
+    P.adj_idx = a->adj_idx;
+    P.k = a->k;
+    P.basis_kind = kind;
+    P.eps = a->basis.eps;
+    P.n_cols = op.n_cols;
+    P.n_terms = op.n_terms;
+    for (int t = 0; t < op.n_terms; ++t) {
+        P.terms[t].col = op.terms[t].col;
+        P.terms[t].kind = op.terms[t].kind;
+        P.terms[t].d1 = op.terms[t].dim1;
+        P.terms[t].d2 = op.terms[t].dim2;
+        P.terms[t].coeff = op.terms[t].coeff;
+    }
+    P.poly_deg = deg;
+    P.n_mono = deg >= 0 ? rbfx::fill_mono_table(a->dim, deg, P.mono_e) : 0;
+    P.is_boundary = a->is_boundary;
+    P.bnd_index = a->bnd_index;
+    P.bc_alpha = a->bc_alpha;
+    P.bc_beta = a->bc_beta;
+    P.normals = a->normals;
+    P.out_ptr = a->out_ptr;
+    P.out_col = a->out_col;
+    P.out_val = a->out_val;
+    P.out_nnz = a->out_nnz > 0 ? a->out_nnz : a->n_eval * (int64_t)a->k;
+    P.status = status;
+    P.refine_steps = a->refine_steps > 0 ? a->refine_steps : 4;
+    P.accept_tol = a->accept_tol > 0 ? a->accept_tol : 1e-12;
+    P.ok_status = RBFX_ST_OK;
+
+    const int nmax = a->k + M;
+    if (a->precision == RBFX_PREC_FP64) {
+        HIPCHK(rbfx::launch_build(P, a->dim, nmax, false, a->n_eval, st));
+    } else {
+        HIPCHK(grow((void**)&ds.fail_list, ds.cap_fail, sizeof(int32_t) * (size_t)a->n_eval));
+        if (!ds.fail_count) HIPCHK(hipMalloc((void**)&ds.fail_count, sizeof(int32_t)));
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
+        P.fail_list = ds.fail_list;
+        P.fail_count = ds.fail_count;
+        HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, a->n_eval, st));
+        // FP64 re-solve of the stencils whose refinement did not converge
+        rbfx::BuildDev F = P;
+        F.fail_list = nullptr;
+        F.fail_count = nullptr;
+        F.list = ds.fail_list;
+        F.list_len = ds.fail_count;
+        F.ok_status = RBFX_ST_FALLBACK64;
+        HIPCHK(rbfx::launch_build(F, a->dim, nmax, false, a->n_eval, st));
+    }
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ds.hist, 16);
+    const int64_t hb = std::min<int64_t>((a->n_eval + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_hist16, dim3((unsigned)hb), dim3(256), 0, st, status, a->n_eval, ds.hist);
+    unsigned long long h[16];
+    HIPCHK(hipMemcpyAsync(h, ds.hist, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < 16; ++i) g_counts[i] = (int64_t)h[i];
+    if (h[RBFX_ST_TOO_LARGE])
+        return set_err(RBFX_EUNSUPPORTED, "%llu stencils exceed the device maximum k+m=%d", h[RBFX_ST_TOO_LARGE], RBFX_MAX_N);
+    if (h[RBFX_ST_EVAL_NOT_FOUND])
+        return set_err(RBFX_EINVAL, "evaluation point not found in stencil data (%llu Hermite stencils)",
+                       h[RBFX_ST_EVAL_NOT_FOUND]);
+    if (h[RBFX_ST_SINGULAR])
+        return set_err(RBFX_ESINGULAR, "%llu stencil systems are singular", h[RBFX_ST_SINGULAR]);
+    g_err.clear();
+    return RBFX_OK;
+}
+
+int rbfx_spmv(const rbfx_spmv_args* a)
+{
+    if (!a) return set_err(RBFX_EINVAL, "rbfx_spmv: null args");
+    if (!a->col_idx || !a->vals || !a->x || !a->y) return set_err(RBFX_EINVAL, "rbfx_spmv: null pointer argument");
+    if (a->n_fields < 1 || a->n_fields > 64) return set_err(RBFX_EUNSUPPORTED, "rbfx_spmv: n_fields=%d out of range 1..64", a->n_fields);
+    if (a->n_valarrays < 1) return set_err(RBFX_EINVAL, "rbfx_spmv: n_valarrays must be >= 1");
+    if (!a->row_ptr && a->k < 1) return set_err(RBFX_EINVAL, "rbfx_spmv: k must be >= 1 without row_ptr");
+    if (a->n_rows == 0) return RBFX_OK;
+    hipStream_t st = (hipStream_t)a->stream;
+    rbfx::SpmvDev P;
+    P.n_rows = a->n_rows;
+    P.row_ptr = a->row_ptr;
+    P.k = a->k;
+    P.col_idx = a->col_idx;
+    P.nnz = a->nnz > 0 ? a->nnz : a->n_rows * (int64_t)a->k;
+    P.n_fields = a->n_fields;
+    P.x = a->x;
+    P.x_ld_point = a->x_ld_point;
+    P.x_ld_field = a->x_ld_field;
+    P.y_ld_point = a->y_ld_point;
+    P.y_ld_field = a->y_ld_field;
+    P.y_ld_val = a->y_ld_val;
+    P.alpha = a->alpha;
+    P.beta = a->beta;
+    for (int v0 = 0; v0 < a->n_valarrays; v0 += 4) {
+        const int nv = std::min(4, a->n_valarrays - v0);
+        P.n_valarrays = nv;
+        P.vals = a->vals + (int64_t)v0 * P.nnz;
+        P.y = a->y + (int64_t)v0 * a->y_ld_val;
+        HIPCHK(rbfx::launch_spmv(P, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+}  // extern "C"

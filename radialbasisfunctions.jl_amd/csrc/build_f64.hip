@@ -1,0 +1,11 @@
+// Instantiates the stencil build kernel for float factorisation, up to 64-row systems.
+#include "build_kernel.h"
+
+namespace rbfx {
+
+hipError_t launch_build_f64(const BuildDev& P, int dim, int64_t n_work, hipStream_t s)
+{
+    return dispatch_dim<float, 64>(P, dim, n_work, s);
+}
+
+}  // namespace rbfx

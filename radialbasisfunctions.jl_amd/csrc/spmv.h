@@ -1,0 +1,26 @@
+// Internal interface of the CSR SpMV.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rbfx {
+
+struct SpmvDev {
+    int64_t n_rows;
+    const int64_t* row_ptr;
+    int k;
+    const int32_t* col_idx;
+    const double* vals;
+    int64_t nnz;
+    int n_valarrays;
+    int n_fields;
+    const double* x;
+    int64_t x_ld_point, x_ld_field;
+    double* y;
+    int64_t y_ld_point, y_ld_field, y_ld_val;
+    double alpha, beta;
+};
+
+hipError_t launch_spmv(const SpmvDev& P, hipStream_t s);
+
+}  // namespace rbfx

@@ -1,0 +1,294 @@
+"""GPU parity tests of the stencil build, k-NN and apply against the oracle.
+
+Tolerances (FP64 plain method): per stencil ||w_gpu - w_oracle||_inf / ||w_oracle||_inf
+<= 1e-10 against the oracle's centred, radius-scaled arithmetic (the same formulation
+in a different pivot order), and <= 1e-10 for 99.5% of stencils against the
+reference's uncentred arithmetic with every stencil <= 1e-7 (that arithmetic itself
+loses digits on ill-conditioned stencils)."""
+import json
+import pathlib
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_goldens.json").read_text())
+
+
+def rel_err_rows(wg, wr, k):
+    wg, wr = wg.reshape(-1, k), wr.reshape(-1, k)
+    return np.abs(wg - wr).max(1) / np.maximum(np.abs(wr).max(1), 1e-300)
+
+
+def _op(O, n_cols, terms):
+    return O.abi.make_operator(n_cols, terms)
+
+
+def _gpu_weights(op):
+    vals = op.vals.cpu().numpy()
+    return vals
+
+
+# ----------------------------------------------------------------- k-NN
+@pytest.mark.parametrize("dim,k,nq", [(2, 20, 3000), (3, 40, 2000), (2, 60, 1000), (1, 8, 500), (3, 30, 2500)])
+def test_knn_matches_bruteforce(RBF, O, gpu, dim, k, nq):
+    rng = np.random.default_rng(100 + dim * k)
+    pts = rng.random((6000, dim))
+    q = rng.random((nq, dim)) * 1.2 - 0.1  # some queries outside the data box
+    got = RBF.find_neighbors(pts, q, k=k)
+    ref = O.knn(pts, q, k)
+    assert np.array_equal(got, ref)
+
+
+def test_knn_self_first_and_goldens(RBF, gpu):
+    g = GOLD["knn_small"]
+    pts = np.array(g["points"])
+    adj = RBF.find_neighbors(pts, k=g["k"])
+    assert all(adj[i, 0] == i for i in range(len(pts)))
+    assert adj[0, 1] in g["second_of_0_in"]
+    ev = g["eval"]
+    assert RBF.find_neighbors(pts, np.array(ev["points"]), k=ev["k"])[0, 0] == ev["first"]
+
+
+def test_knn_clustered_density(RBF, O, gpu):
+    # x100 density contrast: clusters plus sparse background
+    rng = np.random.default_rng(7)
+    pts = np.concatenate([rng.normal(0.3, 0.01, (4000, 2)), rng.random((1000, 2))])
+    got = RBF.find_neighbors(pts, k=24)
+    assert np.array_equal(got, O.knn(pts, pts, 24))
+
+
+def test_knn_ties_on_grid(RBF, O, gpu):
+    # lattice points: many exactly equal distances, ordered by lower index
+    g = np.arange(30) / 29.0
+    pts = np.array([[x, y] for x in g for y in g])
+    got = RBF.find_neighbors(pts, k=13)
+    assert np.array_equal(got, O.knn(pts, pts, 13))
+
+
+# ----------------------------------------------------------------- builds
+def _c1_points(n=10_000, seed=1):
+    return np.random.default_rng(seed).random((n, 2))
+
+
+def test_c1_laplacian_fp64_parity(RBF, O, gpu):
+    pts = _c1_points()
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20)
+    adj = op.adj_idx.cpu().numpy()
+    assert np.array_equal(adj, O.knn(pts, pts, 20))
+    wg = _gpu_weights(op)[0]
+    lap = _op(O, 1, [(0, O.abi.OP_LAP, 0, 0, 1.0)])
+    b = O.basis_struct(3, 2)
+    _, _, ws, _, _ = O.build(pts, pts, adj, b, lap, scaled=True)
+    _, _, wr, _, _ = O.build(pts, pts, adj, b, lap, scaled=False)
+    es = rel_err_rows(wg, ws[0], 20)
+    er = rel_err_rows(wg, wr[0], 20)
+    assert es.max() <= 1e-10, es.max()
+    assert np.quantile(er, 0.995) <= 1e-10 and er.max() <= 1e-7, (np.quantile(er, 0.995), er.max())
+    # applied operator vs analytic: f = sin4x cos3y, lap f = -25 f
+    f = np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1])
+    out = op(f)
+    W = op.weights()
+    assert np.all(np.abs(out - W @ f) <= 1e-13 * (abs(W) @ np.abs(f)))
+    nrmse = np.sqrt(np.mean((out + 25 * f) ** 2)) / np.sqrt(np.mean((25 * f) ** 2))
+    assert nrmse < 0.05
+
+
+BASES = [("PHS1", 1, 1.0), ("PHS3", 3, 1.0), ("PHS5", 5, 1.0), ("PHS7", 7, 1.0), ("IMQ", 10, 3.0),
+         ("Gaussian", 11, 2.0)]
+
+
+def _basis_obj(RBF, name, eps, deg):
+    if name.startswith("PHS"):
+        return RBF.PHS(int(name[3]), poly_deg=deg)
+    return (RBF.IMQ if name == "IMQ" else RBF.Gaussian)(eps, poly_deg=deg)
+
+
+@pytest.mark.parametrize("name,kind,eps", BASES)
+@pytest.mark.parametrize("dim,deg,k", [(2, 2, 20), (3, 1, 16), (2, 0, 12), (1, 2, 8), (3, 3, 40), (2, 3, 30)])
+def test_build_parity_bases(RBF, O, gpu, name, kind, eps, dim, deg, k):
+    if name == "PHS1" and deg >= 2:
+        pytest.skip("PHS1 with quadratic augmentation is not a stable pairing")
+    rng = np.random.default_rng(kind * 10 + dim * 3 + deg)
+    n = 1500
+    pts = rng.random((n, dim)) * (0.25 if kind >= 10 else 1.0)
+    basis = _basis_obj(RBF, name, eps, deg)
+    L = RBF.Laplacian() + 0.5 * RBF.Partial(1, 1)
+    op = RBF.RadialBasisOperator(L, pts, basis=basis, k=k)
+    adj = op.adj_idx.cpu().numpy()
+    wg = _gpu_weights(op)[0]
+    terms = [(0, O.abi.OP_LAP, 0, 0, 1.0), (0, O.abi.OP_D1, 0, 0, 0.5)]
+    _, _, ws, st, rc = O.build(pts, pts, adj, O.basis_struct(kind, deg, eps), _op(O, 1, terms), scaled=True)
+    assert rc == 0
+    e = rel_err_rows(wg, ws[0], k)
+    assert np.quantile(e, 0.999) <= 1e-10 and e.max() <= 1e-8, (np.quantile(e, 0.999), e.max())
+
+
+@pytest.mark.parametrize("which", ["d1", "d2", "mixed", "regrid", "grad", "hess3", "sum"])
+def test_build_parity_operators(RBF, O, gpu, which):
+    rng = np.random.default_rng(11)
+    dim = 3 if which == "hess3" else 2
+    pts = rng.random((2500, dim))
+    b = RBF.PHS(5, poly_deg=3) if dim == 3 else RBF.PHS(3, poly_deg=2)
+    k = 40 if dim == 3 else 20
+    ev = None
+    if which == "d1":
+        L, terms, nc = RBF.Partial(1, 2), [(0, 1, 1, 0, 1.0)], 1
+    elif which == "d2":
+        L, terms, nc = RBF.Partial(2, 1), [(0, 2, 0, 0, 1.0)], 1
+    elif which == "mixed":
+        L, terms, nc = RBF.MixedPartial(1, 2), [(0, 3, 0, 1, 1.0)], 1
+    elif which == "regrid":
+        ev = rng.random((900, dim))
+        L, terms, nc = RBF.Regrid(), [(0, 0, 0, 0, 1.0)], 1
+    elif which == "grad":
+        L, terms, nc = RBF.Jacobian(2), [(0, 1, 0, 0, 1.0), (1, 1, 1, 0, 1.0)], 2
+    elif which == "hess3":
+        L = RBF.Hessian(3)
+        pairs = [(i, j) for i in range(3) for j in range(i, 3)]
+        terms = [(c, 2 if i == j else 3, i, j, 1.0) for c, (i, j) in enumerate(pairs)]
+        nc = 6
+    else:
+        L = 2.0 * RBF.Partial(2, 1) + 0.5 * RBF.Partial(2, 2) - RBF.Identity()
+        terms, nc = [(0, 2, 0, 0, 2.0), (0, 2, 1, 0, 0.5), (0, 0, 0, 0, -1.0)], 1
+    op = RBF.RadialBasisOperator(L, pts, eval_points=ev, basis=b, k=k)
+    adj = op.adj_idx.cpu().numpy()
+    evp = pts if ev is None else ev
+    _, _, ws, st, rc = O.build(pts, evp, adj, O.basis_struct(b.kind, b.poly_deg), _op(O, nc, terms), scaled=True)
+    wg = _gpu_weights(op)
+    for c in range(nc):
+        e = rel_err_rows(wg[c], ws[c], k)
+        assert e.max() <= 1e-9, (c, np.quantile(e, 0.99), e.max())
+
+
+def test_fp32_refined_matches_fp64(RBF, O, gpu):
+    pts = _c1_points(6000, seed=3)
+    o64 = RBF.laplacian(pts, k=20)
+    o32 = RBF.laplacian(pts, k=20, adjl=o64.adj_idx.cpu().numpy(), precision="fp32_refined")
+    counts = o32.status_counts
+    assert counts[RBF._abi.ST_REFINED] + counts[RBF._abi.ST_FALLBACK64] == len(pts)
+    e = rel_err_rows(_gpu_weights(o32)[0], _gpu_weights(o64)[0], 20)
+    assert e.max() <= 1e-9, e.max()
+
+
+def test_variable_size_stencils(RBF, O, gpu):
+    rng = np.random.default_rng(21)
+    pts = rng.random((3000, 2))
+    kmax = 30
+    full = O.knn(pts, pts, kmax)
+    ks = rng.integers(12, kmax + 1, size=len(pts))
+    adjl = [full[i, :ks[i]] for i in range(len(pts))]
+    op = RBF.laplacian(pts, adjl=adjl)
+    ptr = np.zeros(len(pts) + 1, dtype=np.int64)
+    np.cumsum(ks, out=ptr[1:])
+    idx = np.concatenate(adjl).astype(np.int32)
+    _, _, ws, _, rc = O.build(pts, pts, (ptr, idx), O.basis_struct(3, 2), _op(O, 1, [(0, 4, 0, 0, 1.0)]),
+                              scaled=True)
+    wg = _gpu_weights(op)[0]
+    for i in range(0, len(pts), 7):
+        a, b = ptr[i], ptr[i + 1]
+        assert np.abs(wg[a:b] - ws[0][a:b]).max() <= 1e-10 * np.abs(ws[0][a:b]).max()
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_hermite_domain12(RBF, O, gpu, case):
+    H = GOLD["hermite_domain_12"]
+    c = H["cases"][case]
+    pts = np.array(H["points"])
+    isb = np.array(H["is_boundary"])
+    bcs = [RBF.BoundaryCondition(a, b) for a, b in c["bcs"]]
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2),
+                       hermite=dict(is_boundary=isb, bc=bcs, normals=np.array(H["normals"])))
+    u = pts[:, 0] ** 2 + pts[:, 1] ** 2 if c["field"] == "x2+y2" else pts[:, 1] ** 2
+    res = op(u)
+    assert np.abs(res[~isb] - c["interior_laplacian"]).max() < c["tol"]
+    adj = op.adj_idx.cpu().numpy()
+    al = np.array([b[0] for b in c["bcs"]], float)
+    be = np.array([b[1] for b in c["bcs"]], float)
+    ptr, col, ws, st, rc = O.build(pts, pts, adj, O.basis_struct(3, 2), _op(O, 1, [(0, 4, 0, 0, 1.0)]),
+                                   hermite=(isb, al, be, np.array(H["normals"])), scaled=True)
+    W = op.weights()
+    assert np.array_equal(W.indptr, ptr) and np.array_equal(W.indices, col)
+    assert np.abs(W.data - ws[0]).max() <= 1e-10 * np.abs(ws[0]).max()
+
+
+def test_hermite_manufactured_441(RBF, O, gpu):
+    from test_oracle import _unit_square_441
+
+    pts, isb, nrm, bcs = _unit_square_441()
+    adjl = O.knn(pts, pts, 15)
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), adjl=adjl,
+                       hermite=dict(is_boundary=isb, bc=[RBF.BoundaryCondition(a, b) for a, b in bcs],
+                                    normals=nrm))
+    u = pts[:, 0] ** 2 + pts[:, 1] ** 2 + 1
+    un = (2 * pts[isb] * nrm).sum(1)
+    ub = np.where((bcs[:, 0] == 1) & (bcs[:, 1] == 0), u[isb], bcs[:, 0] * u[isb] + bcs[:, 1] * un)
+    uu = u.copy()
+    uu[isb] = ub
+    res = op(uu)
+    assert np.abs(res[~isb] - 4.0).max() < 1e-10
+    assert np.abs(res[isb] - ub).max() < 1e-10
+
+
+def test_singular_stencil_raises(RBF, gpu):
+    pts = np.random.default_rng(0).random((50, 2))
+    pts[1] = pts[0]  # duplicate point -> singular collocation matrix
+    with pytest.raises(RBF.SingularException):
+        RBF.laplacian(pts, k=12)
+
+
+def test_normal_form_guard(RBF, gpu):
+    H = GOLD["hermite_domain_12"]
+    pts = np.array(H["points"])
+    herm = dict(is_boundary=np.array(H["is_boundary"]), bc=[RBF.Neumann(), RBF.Neumann()],
+                normals=np.array(H["normals"]))
+    with pytest.raises(RBF.ArgumentError):
+        RBF.laplacian(pts, basis=RBF.IMQ(1.0), hermite=herm)
+    with pytest.raises(RBF.DimensionMismatch):
+        RBF.laplacian(pts, hermite=dict(is_boundary=herm["is_boundary"][:5], bc=herm["bc"],
+                                        normals=herm["normals"]))
+
+
+# ----------------------------------------------------------------- apply
+@pytest.mark.parametrize("F", [1, 3, 16])
+def test_spmv_fields(RBF, O, gpu, F):
+    rng = np.random.default_rng(F)
+    pts = rng.random((4000, 3))
+    ev = rng.random((3000, 3))
+    op = RBF.regrid(pts, ev, basis=RBF.IMQ(3.0, poly_deg=1), k=30)
+    X = rng.standard_normal((4000, F)) if F > 1 else rng.standard_normal(4000)
+    Y = op(X)
+    W = op.weights()
+    assert np.all(np.abs(Y - W @ X) <= 1e-13 * (abs(W) @ np.abs(X)) + 1e-300)
+
+
+def test_spmv_multi_valarrays_and_mul(RBF, gpu):
+    rng = np.random.default_rng(2)
+    pts = rng.random((3000, 2))
+    g = RBF.gradient(pts)
+    f = np.sin(3 * pts[:, 0]) + pts[:, 1] ** 2
+    G = g(f)
+    Wx, Wy = g.weights()
+    assert np.all(np.abs(G[:, 0] - Wx @ f) <= 1e-13 * (abs(Wx) @ np.abs(f)))
+    assert np.all(np.abs(G[:, 1] - Wy @ f) <= 1e-13 * (abs(Wy) @ np.abs(f)))
+    lap = RBF.laplacian(pts)
+    y = rng.standard_normal(3000)
+    y0 = y.copy()
+    lap.mul(y, f, 2.0, 0.5)
+    W = lap.weights()
+    assert np.all(np.abs(y - (2.0 * (W @ f) + 0.5 * y0)) <= 1e-13 * (2.0 * (abs(W) @ np.abs(f)) + np.abs(y0)))
+
+
+def test_built_operator_algebra(RBF, gpu):
+    rng = np.random.default_rng(4)
+    pts = rng.random((2000, 2))
+    dxx = RBF.partial(pts, 2, 1)
+    dyy = RBF.partial(pts, 2, 2)
+    lap_sum = (RBF.Partial(2, 1) + RBF.Partial(2, 2))(pts)
+    comb = dxx + dyy
+    assert np.allclose(comb.weights().toarray(), lap_sum.weights().toarray(), rtol=1e-10, atol=1e-8)
+    sc = 2.5 * dxx
+    assert np.allclose(sc.weights().toarray(), 2.5 * dxx.weights().toarray(), rtol=1e-12)
+    other = RBF.partial(pts[:1000], 2, 1)
+    with pytest.raises(RBF.ArgumentError):
+        dxx + other
