@@ -1,30 +1,37 @@
-// Fused RBF-FD stencil assemble-and-solve for gfx950.
+// Fused RBF-FD stencil assemble-and-solve for gfx950 (kernel template).
 //
 // Replaces the per-stencil hot loop of build_weights_kernel
 // (src/solve/execution.jl:186-265): _build_collocation_matrix! (assembly.jl:95-115),
 // _build_rhs! (assembly.jl:127-185), the Hermite dispatch (assembly.jl:240-467) and
 // _solve_system! = bunchkaufman!(Symmetric(A,:U), true) (assembly.jl:220-222).
 //
-// Mapping: one 64-lane wavefront per stencil. The saddle-point system
+// Mapping: a group of NMAX lanes per stencil (two stencils per wavefront when
+// NMAX = 32, one when NMAX = 64). The saddle-point system
 //     [ 0   P^T ] [lambda]   [L P (x_e)    ]
 //     [ P   Phi ] [  w   ] = [L phi(x_e, .)]
 // (polynomial rows first) is assembled in a centred, radius-scaled frame directly
-// into registers: lane L owns row L, r[j] holds A[L][j] (static j), and the operator
-// columns ride along as extra register columns r[NMAX + c] (augmented system).
+// into registers: lane li of a group owns row li, r[j] holds A[li][j], and the
+// operator columns ride along as register columns r[NMAX + c] (augmented system).
+//
+// Registers are only ever indexed statically. Where an index is uniform over a group
+// (the assembly column, a queried column, a pivot), the register is selected through
+// a switch on a scalar value, which the compiler lowers to scalar branches; a wave
+// holding two groups runs the switch once per group.
 //
 // Factorisation: symmetric-indefinite LDL^T with rook pivoting (the pivot tests of
-// Ashcraft, Grimes & Lewis 1998, as LAPACK ?sytrf_rook uses). Rows never move
-// between lanes; a "query" of column j broadcasts row j through LDS (by symmetry it
-// is column j) and takes a cross-lane argmax. The pivot row(s) go to LDS, where they
-// stay as the rows of U = D L^T, and every still-active lane applies the rank-1 or
-// rank-2 update to its register row. The step loop is a runtime loop, so registers
-// are only ever indexed statically. Back substitution is right-looking: when x_p is
-// known every lane that pivoted earlier reads its own U-row entry for column p.
+// Ashcraft, Grimes & Lewis 1998, as LAPACK ?sytrf_rook uses). Rows never move between
+// lanes. Column queries read each lane's own register (A is symmetric), a group
+// argmax (DPP rotations inside 16-lane rows, one cross-row exchange) picks the largest,
+// and the pivot row(s) go through LDS as the broadcast source of the rank-1/rank-2
+// update. A lane's registers freeze when its row pivots, so they keep its row of U
+// (U = D L^T, plus the eliminated right-hand sides) for the back substitution, which
+// is right-looking: once x_p is known every lane that pivoted earlier subtracts its
+// own U entry for column p.
 //
 // FP32 mode keeps an FP64 copy of the row and of the right-hand sides: the FP32
-// solution is refined with FP64 residuals b - A x and FP32 corrections that replay
-// the stored multipliers; stencils whose correction does not settle are queued for
-// the FP64 kernel.
+// solution is refined with FP64 residuals b - A x and FP32 corrections that replay the
+// multipliers (kept per lane, indexed by pivot row); stencils whose correction does
+// not settle are queued for the FP64 kernel.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -32,49 +39,173 @@
 #include "build.h"
 #include "device_math.h"
 
-// waves per SIMD the register allocation targets (64-row systems hold twice the
-// registers of 32-row ones)
-template <typename T, int NMAX>
-struct BuildOccupancy {
-    static constexpr int waves = (NMAX <= 32) ? 2 : 1;
-};
-
 namespace rbfx {
 
-// ---------------------------------------------------------------- wave helpers
+// ---------------------------------------------------------------- lane helpers
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
-__device__ __forceinline__ double readlane(double v, int lane)
+__device__ __forceinline__ int rdlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ __forceinline__ float rdlane(float v, int l)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ double rdlane(double v, int l)
 {
     const unsigned long long u = (unsigned long long)__double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(unsigned)u, lane);
-    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(u >> 32), lane);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+    const int lo = dpp_i<CTRL>((int)(unsigned)u), hi = dpp_i<CTRL>((int)(unsigned)(u >> 32));
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-__device__ __forceinline__ float readlane(float v, int lane)
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
 {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+    return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
 }
-__device__ __forceinline__ int readlane(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
-
-__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 template <typename T>
-__device__ __forceinline__ void wave_argmax(T& v, int& idx)
+__device__ __forceinline__ void argmax_step(T& v, int& i, T ov, int oi)
 {
+    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+}
+
+// all-reduce (max value, lowest index on ties) over aligned groups of NMAX lanes;
+// every lane of the wave must be active
+template <int NMAX, typename T>
+__device__ __forceinline__ void group_argmax(T& v, int& i)
+{
+    // row_ror:1,2,4,8 inside each 16-lane row (DPP, no LDS round trip)
+    argmax_step(v, i, dpp<0x121>(v), dpp_i<0x121>(i));
+    argmax_step(v, i, dpp<0x122>(v), dpp_i<0x122>(i));
+    argmax_step(v, i, dpp<0x124>(v), dpp_i<0x124>(i));
+    argmax_step(v, i, dpp<0x128>(v), dpp_i<0x128>(i));
 #pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-        const T ov = __shfl_xor(v, m, 64);
-        const int oi = __shfl_xor(idx, m, 64);
-        if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
+    for (int m = 16; m < NMAX; m <<= 1) argmax_step(v, i, (T)__shfl_xor(v, m, 64), __shfl_xor(i, m, 64));
+}
+
+template <int NMAX>
+__device__ __forceinline__ double group_max(double v)
+{
+    v = fmax(v, dpp<0x121>(v));
+    v = fmax(v, dpp<0x122>(v));
+    v = fmax(v, dpp<0x124>(v));
+    v = fmax(v, dpp<0x128>(v));
+#pragma unroll
+    for (int m = 16; m < NMAX; m <<= 1) v = fmax(v, __shfl_xor(v, m, 64));
+    return v;
+}
+
+// value of lane `src` of this lane's group; src must be uniform over each group
+template <int NMAX, typename T>
+__device__ __forceinline__ T group_bcast(T v, int src)
+{
+    if constexpr (NMAX == 64) {
+        return rdlane(v, __builtin_amdgcn_readfirstlane(src));
+    } else {
+        const int a = rdlane(src, 0), b = rdlane(src, 32);
+        const T va = rdlane(v, a), vb = rdlane(v, 32 + b);
+        return lane_id() >= 32 ? vb : va;
     }
 }
 
-__device__ __forceinline__ double wave_max(double v)
+// bits of a wave ballot that belong to this lane's group
+template <int NMAX>
+__device__ __forceinline__ unsigned long long group_bits(unsigned long long m)
 {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+    if constexpr (NMAX == 64) return m;
+    const int g = lane_id() / NMAX;
+    return (m >> (g * NMAX)) & ((1ull << NMAX) - 1ull);
+}
+
+// ------------------------------------------------ register selection by scalar index
+// r[j] for a wave-uniform j in [0, 64): a switch over a scalar value (scalar branches)
+#define RBFX_CASES(X)                                                                                 \
+    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) \
+    X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32) X(33)   \
+    X(34) X(35) X(36) X(37) X(38) X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49)   \
+    X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58) X(59) X(60) X(61) X(62) X(63)
+
+template <int N, typename T>
+__device__ __forceinline__ T reg_get(const T (&r)[N], int j)
+{
+    T v = (T)0;
+    switch (j) {
+#define RBFX_GET(q)                    \
+    case q:                            \
+        if constexpr (q < N) v = r[q]; \
+        break;
+        RBFX_CASES(RBFX_GET)
+#undef RBFX_GET
+    default: break;
+    }
     return v;
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void reg_set(T (&r)[N], int j, T v)
+{
+    switch (j) {
+#define RBFX_SET(q)                    \
+    case q:                            \
+        if constexpr (q < N) r[q] = v; \
+        break;
+        RBFX_CASES(RBFX_SET)
+#undef RBFX_SET
+    default: break;
+    }
+}
+
+// r[idx] where idx is uniform over each group of NMAX lanes
+template <int NMAX, int N, typename T>
+__device__ __forceinline__ T group_reg_get(const T (&r)[N], int idx)
+{
+    if constexpr (NMAX == 64) {
+        return reg_get<N, T>(r, __builtin_amdgcn_readfirstlane(idx));
+    } else {
+        const int a = rdlane(idx, 0), b = rdlane(idx, 32);
+        const T va = reg_get<N, T>(r, a), vb = reg_get<N, T>(r, b);
+        return lane_id() >= 32 ? vb : va;
+    }
+}
+
+// r[j] = v on the lanes where pred holds (j wave-uniform; a select, no lane divergence)
+template <int N, typename T>
+__device__ __forceinline__ void reg_set_if(T (&r)[N], int j, T v, bool pred)
+{
+    switch (j) {
+#define RBFX_SETIF(q)                                 \
+    case q:                                           \
+        if constexpr (q < N) r[q] = pred ? v : r[q]; \
+        break;
+        RBFX_CASES(RBFX_SETIF)
+#undef RBFX_SETIF
+    default: break;
+    }
+}
+
+// r[idx] = v on the lanes where pred holds; idx uniform over each group
+template <int NMAX, int N, typename T>
+__device__ __forceinline__ void group_reg_set(T (&r)[N], int idx, T v, bool pred)
+{
+    if constexpr (NMAX == 64) {
+        reg_set_if<N, T>(r, __builtin_amdgcn_readfirstlane(idx), v, pred);
+    } else {
+        const int a = rdlane(idx, 0), b = rdlane(idx, 32);
+        reg_set_if<N, T>(r, a, v, pred && lane_id() < 32);
+        reg_set_if<N, T>(r, b, v, pred && lane_id() >= 32);
+    }
 }
 
 // ---------------------------------------------------------------- monomials
@@ -136,511 +267,567 @@ __device__ __forceinline__ double term_scale(const TermDev& t, double hinv)
     return t.coeff * (t.kind == RBFX_OP_VALUE ? 1.0 : (t.kind == RBFX_OP_D1 ? hinv : hinv * hinv));
 }
 
+// per-stencil tables in LDS
+template <typename T, int NMAX, int DIM, int UW>
+struct StencilLds {
+    double xi[NMAX][DIM];  // scaled offsets of the stencil points
+    double ba[NMAX], bb[NMAX], bn[NMAX][DIM];
+    T prow[2][2][UW];  // pivot rows, double-buffered by step parity
+    int bt[NMAX];      // 0 interior, 1 Dirichlet, 2 Neumann/Robin (Hermite stencils)
+    int piv[NMAX], blk[NMAX];
+};
+
 // ------------------------------------------------------- back substitution
-// Solves U x = y given the pivot sequence (s_piv, s_blk), the U rows in s_U and
-// each lane's pivot slot. y: this lane's forward-eliminated RHS entry.
-template <typename T, int NMAX>
-__device__ __forceinline__ T back_substitute(const T (*s_U)[NMAX], const int* s_piv, const int* s_blk, const int n,
-                                             const int myslot, const T diag, T y)
+// Solves U x = y (RC right-hand sides at once) for this lane's group. Row p of U is
+// the frozen register row of the lane that pivoted at slot s (s_piv[s] = p), with the
+// block structure in s_blk (1: 1x1, 2: first slot of a 2x2 block, 0: its second slot).
+template <int NMAX, int RC, typename T, int N>
+__device__ __forceinline__ void back_substitute(const T (&r)[N], const int* s_piv, const int* s_blk, int nslots,
+                                                int nslots_wave, int myslot, T diag, int n_cols, const T (&y)[RC],
+                                                T (&x)[RC])
 {
+    const int li = lane_id() % NMAX;
     const T dinv = (T)1 / diag;
-    T acc = (T)0, x = (T)0;
-    const int L = lane_id();
-    const int ms = myslot < NMAX ? myslot : 0;
-    for (int s = n - 1; s >= 0; --s) {
-        const int blk = uniform(s_blk[s]);
-        if (blk == 1) {
-            const int p = uniform(s_piv[s]);
-            const T xp = readlane((y - acc) * dinv, p);
-            if (L == p) x = xp;
-            if (myslot < s) acc += s_U[ms][p] * xp;
-        } else if (blk == 0 && s > 0) {
-            // second slot of a 2x2 block (s-1, s)
-            const int p = uniform(s_piv[s - 1]), q = uniform(s_piv[s]);
-            const T a = readlane(diag, p), bq = s_U[s - 1][q], c = readlane(diag, q);
-            const T y0 = readlane(y - acc, p), y1 = readlane(y - acc, q);
-            const T idet = (T)1 / (a * c - bq * bq);
-            const T x0 = (y0 * c - y1 * bq) * idet, x1 = (y1 * a - y0 * bq) * idet;
-            if (L == p) x = x0;
-            if (L == q) x = x1;
-            if (myslot < s - 1) acc += s_U[ms][p] * x0 + s_U[ms][q] * x1;
-            --s;
+    T acc[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) {
+        acc[c] = (T)0;
+        x[c] = (T)0;
+    }
+    for (int s = nslots_wave - 1; s >= 0; --s) {
+        const int blk = s < nslots ? s_blk[s] : -1;
+        const int bprev = (s < nslots && s > 0) ? s_blk[s - 1] : -1;
+        const bool one = blk == 1, two = (blk == 0) && (bprev == 2);
+        const int p = one ? s_piv[s] : (two ? s_piv[s - 1] : 0);
+        const int q = two ? s_piv[s] : 0;
+        const T up = group_reg_get<NMAX>(r, p);
+        const T uq = group_reg_get<NMAX>(r, q);
+        const T dp = group_bcast<NMAX>(diag, p), dq = group_bcast<NMAX>(diag, q);
+        const T bq = group_bcast<NMAX>(uq, p);  // U[p][q]
+        const T idet = (T)1 / (dp * dq - bq * bq);
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            if (c < n_cols) {
+                const T res = y[c] - acc[c];
+                const T xp = group_bcast<NMAX>(res * dinv, p);
+                const T y0 = group_bcast<NMAX>(res, p), y1 = group_bcast<NMAX>(res, q);
+                const T x0 = two ? (y0 * dq - y1 * bq) * idet : xp;
+                const T x1 = two ? (y1 * dp - y0 * bq) * idet : (T)0;
+                x[c] = ((one || two) && li == p) ? x0 : ((two && li == q) ? x1 : x[c]);
+                const bool upd = one ? (myslot < s) : (two && myslot < s - 1);
+                const T dacc = two ? up * x0 + uq * x1 : up * x0;
+                acc[c] += upd ? dacc : (T)0;
+            }
         }
     }
-    return x;
 }
 
-// forward replay of the stored multipliers on a new RHS (FP32 refinement)
-template <typename T>
-__device__ __forceinline__ T forward_replay(const T (*s_Lm)[64], const int* s_piv, const int* s_blk, const int n,
-                                            const int myslot, T y)
+// forward replay of the multipliers (indexed by pivot row) on a new right-hand side
+template <int NMAX, typename T, int N>
+__device__ __forceinline__ T forward_replay(const T (&Lm)[N], const int* s_piv, const int* s_blk, int nslots,
+                                            int nslots_wave, int myslot, T y)
 {
-    const int L = lane_id();
-    for (int s = 0; s < n; ++s) {
-        const int blk = uniform(s_blk[s]);
-        if (blk == 1) {
-            const T yp = readlane(y, uniform(s_piv[s]));
-            if (myslot > s) y -= s_Lm[s][L] * yp;
-        } else if (blk == 2) {
-            const T y0 = readlane(y, uniform(s_piv[s])), y1 = readlane(y, uniform(s_piv[s + 1]));
-            if (myslot > s + 1) y -= s_Lm[s][L] * y0 + s_Lm[s + 1][L] * y1;
-            ++s;
-        }
+    for (int s = 0; s < nslots_wave; ++s) {
+        const int blk = s < nslots ? s_blk[s] : -1;
+        const int p = blk >= 1 ? s_piv[s] : 0;
+        const int q = blk == 2 ? s_piv[s + 1] : 0;
+        const T y0 = group_bcast<NMAX>(y, p), y1 = group_bcast<NMAX>(y, q);
+        const T l0 = group_reg_get<NMAX>(Lm, p), l1 = group_reg_get<NMAX>(Lm, q);
+        if (blk == 1 && myslot > s) y -= l0 * y0;
+        if (blk == 2 && myslot > s + 1) y -= l0 * y0 + l1 * y1;
     }
     return y;
 }
 
+// r += a * u over a broadcast LDS row, in chunks of 8 columns: the scheduling
+// barriers keep at most one chunk of the row in flight, so the register row is
+// never spilled to make room for prefetched operands
+template <int NMAX, int RC, typename T, int N>
+__device__ __forceinline__ void axpy_row(T (&r)[N], T a, const T* u, int n_cols)
+{
+#pragma unroll
+    for (int j0 = 0; j0 < NMAX; j0 += 8) {
+#pragma unroll
+        for (int j = j0; j < j0 + 8; ++j) r[j] = fma(a, u[j], r[j]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int c = 0; c < RC; ++c)
+        if (c < n_cols) r[NMAX + c] = fma(a, u[NMAX + c], r[NMAX + c]);
+}
+
+// waves per SIMD the register allocation targets
+template <typename T, int NMAX, int RC>
+struct BuildOccupancy {
+    static constexpr int waves = sizeof(T) == 4 ? 2 : (NMAX <= 32 ? (RC <= 4 ? 3 : 2) : 2);
+};
+
 // ------------------------------------------------------------------ kernel
-template <typename T, int NMAX, int DIM>
-__global__ __launch_bounds__(64, (BuildOccupancy<T, NMAX>::waves)) void k_build(const BuildDev P)
+template <typename T, int NMAX, int DIM, int RC>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
+                          amdgpu_waves_per_eu(BuildOccupancy<T, NMAX, RC>::waves))) void
+k_build(const BuildDev P)
 {
     constexpr bool REFINE = sizeof(T) == 4;
-    constexpr int RC = RBFX_MAX_COLS;
-    constexpr int PT = NMAX * NMAX / 4;  // >= kk * M for any split kk + M <= NMAX
+    constexpr int G = 64 / NMAX;   // stencils per wavefront
+    constexpr int UW = NMAX + RC;  // register row: matrix entries + right-hand sides
+    constexpr int NL = REFINE ? NMAX : 1;
 
-    __shared__ double s_xi[NMAX][DIM];
-    __shared__ double s_ba[NMAX], s_bb[NMAX], s_bn[NMAX][DIM];
-    __shared__ int s_bt[NMAX];
-    __shared__ int s_piv[NMAX], s_blk[NMAX];
-    __shared__ T s_col[NMAX];
-    __shared__ double s_x[64];
-    // U rows (factorisation) alias the P table (assembly)
-    constexpr int UB = NMAX * NMAX * (int)sizeof(T), PB = PT * 8;
-    __shared__ __attribute__((aligned(16))) unsigned char s_raw[UB > PB ? UB : PB];
-    T(*s_U)[NMAX] = reinterpret_cast<T(*)[NMAX]>(s_raw);
-    double* s_P = reinterpret_cast<double*>(s_raw);
-    __shared__ T s_Lm[REFINE ? NMAX : 1][64];
-    __shared__ double s_b64[REFINE ? RC : 1][64];
+    __shared__ StencilLds<T, NMAX, DIM, UW> s_st[G];
+    __shared__ double s_x[G][NL];
     __shared__ int s_mono[RBFX_MAX_N][3];
     __shared__ TermDev s_terms[RBFX_MAX_TERMS];
 
     const int L = lane_id();
+    const int g = L / NMAX, li = L % NMAX;
+    StencilLds<T, NMAX, DIM, UW>& S = s_st[g];
     const int n_cols = P.n_cols;
     const int M = P.n_mono;
     const int n_terms = P.n_terms;
     const int64_t n_work = P.list ? (int64_t)*P.list_len : P.n_eval;
-    // operator and monomial tables to LDS once per block (keeps them out of SGPRs)
     {
         const int* src = &P.mono_e[0][0];
         for (int i = L; i < RBFX_MAX_N * 3; i += 64) (&s_mono[0][0])[i] = src[i];
         if (L < RBFX_MAX_TERMS) s_terms[L] = P.terms[L];
     }
+
+    const int64_t wi = (int64_t)blockIdx.x * G + g;
+    bool alive = wi < n_work;
+    const int64_t e = alive ? (P.list ? (int64_t)P.list[wi] : wi) : 0;
+    const int64_t s0 = alive ? (P.adj_ptr ? P.adj_ptr[e] : e * (int64_t)P.k) : 0;
+    const int kk = alive ? (P.adj_ptr ? (int)(P.adj_ptr[e + 1] - s0) : P.k) : 0;
+    const int64_t o0 = alive ? (P.out_ptr ? P.out_ptr[e] : e * (int64_t)P.k) : 0;
+    if (alive && (kk < 1 || M + kk > NMAX)) {
+        if (li == 0 && P.status) P.status[e] = RBFX_ST_TOO_LARGE;
+        alive = false;
+    }
+
+    // ---- this lane's stencil point (point rows live at lanes M..M+kk-1 of the group)
+    const int pi = li - M;
+    bool is_pt = alive && pi >= 0 && pi < kk;
+    int gidx = 0;
+    double xe[DIM], D[DIM];
+    double r2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+        xe[d] = alive ? P.eval[e * DIM + d] : 0.0;
+        D[d] = 0.0;
+    }
+    if (is_pt) {
+        gidx = P.adj_idx[s0 + pi];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            D[d] = P.data[(int64_t)gidx * DIM + d] - xe[d];
+            r2 += D[d] * D[d];
+        }
+    }
+
+    // ---- stencil classification (types.jl:208-223)
+    bool hermite = false;
+    int bt = 0;
+    double ba = 0.0, bb = 0.0, bn[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) bn[d] = 0.0;
+    if (P.is_boundary) {
+        const bool ib = is_pt && P.is_boundary[gidx];
+        hermite = group_bits<NMAX>(__ballot(ib)) != 0ull;
+        if (alive && hermite && e < P.n_data && P.is_boundary[e]) {
+            const int bi = P.bnd_index[e];
+            if (P.bc_alpha[bi] == 1.0 && P.bc_beta[bi] == 0.0) {
+                // identity row (fill_dirichlet_entry!, execution.jl:297-303)
+                if (li < n_cols) P.out_val[(int64_t)li * P.out_nnz + o0] = 1.0;
+                if (li == 0) {
+                    if (P.out_col) P.out_col[o0] = (int32_t)e;
+                    if (P.status) P.status[e] = RBFX_ST_DIRICHLET;
+                }
+                alive = false;
+                is_pt = false;
+            }
+        }
+        if (hermite && ib && alive) {
+            const int bi = P.bnd_index[gidx];
+            ba = P.bc_alpha[bi];
+            bb = P.bc_beta[bi];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) bn[d] = P.normals[(int64_t)bi * DIM + d];
+            bt = (ba == 1.0 && bb == 0.0) ? 1 : 2;
+        }
+    }
+
+    // ---- radius-scaled local frame
+    double h = sqrt(group_max<NMAX>(is_pt ? r2 : 0.0));
+    if (!(h > 0.0)) h = 1.0;
+    const double hinv = 1.0 / h;
+    double xi[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xi[d] = D[d] / h;
+    bb = bb / h;  // beta acts through a first derivative
+    if (is_pt) {
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) S.xi[pi][d] = xi[d];
+        S.bt[pi] = bt;
+        if (hermite) {
+            S.ba[pi] = ba;
+            S.bb[pi] = bb;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) S.bn[pi][d] = bn[d];
+        }
+    }
+    int eval_local = 0;
+    {
+        bool eq = is_pt && hermite;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) eq = eq && (D[d] == 0.0);
+        const unsigned long long m = group_bits<NMAX>(__ballot(eq));
+        if (alive && hermite) {
+            if (!m) {
+                if (li == 0 && P.status) P.status[e] = RBFX_ST_EVAL_NOT_FOUND;
+                alive = false;
+                is_pt = false;
+            } else {
+                eval_local = __builtin_ctzll(m) - M;
+            }
+        }
+    }
+    const int n = alive ? M + kk : 0;
+    int n_wave = n;
+#pragma unroll
+    for (int m = NMAX; m < 64; m <<= 1) n_wave = max(n_wave, __shfl_xor(n_wave, m, 64));
+    n_wave = __builtin_amdgcn_readfirstlane(n_wave);
     __syncthreads();
 
-    {
-        const int64_t w = blockIdx.x;
-        if (w >= n_work) return;
-        const int64_t e = P.list ? (int64_t)P.list[w] : w;
-        const int64_t s0 = P.adj_ptr ? P.adj_ptr[e] : e * (int64_t)P.k;
-        const int kk = P.adj_ptr ? (int)(P.adj_ptr[e + 1] - s0) : P.k;
-        const int64_t o0 = P.out_ptr ? P.out_ptr[e] : e * (int64_t)P.k;
-        const int n = M + kk;
-        if (kk < 1 || n > NMAX) {
-            if (L == 0 && P.status) P.status[e] = RBFX_ST_TOO_LARGE;
-            return;
-        }
+    BasisDev B;
+    B.kind = P.basis_kind;
+    B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
 
-        // ---- this lane's stencil point (point rows live at lanes M..M+kk-1)
-        const int pi = L - M;
-        const bool is_pt = pi >= 0 && pi < kk;
-        int g = 0;
-        double xe[DIM], D[DIM];
+    // ---- assembly (FP64): column j of every row in step j (j uniform over the wave)
+    T r[UW];
+    double a64[REFINE ? NMAX : 1];
+    double diag64 = 0.0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) xe[d] = P.eval[e * DIM + d];
-        double r2 = 0.0;
+    for (int j = 0; j < UW; ++j) r[j] = (T)0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) D[d] = 0.0;
-        if (is_pt) {
-            g = P.adj_idx[s0 + pi];
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) {
-                D[d] = P.data[(int64_t)g * DIM + d] - xe[d];
-                r2 += D[d] * D[d];
+    for (int j = 0; j < (REFINE ? NMAX : 1); ++j) a64[j] = 0.0;
+    const bool is_poly = alive && li < M;
+    for (int j = 0; j < n_wave; ++j) {
+        double v = 0.0;
+        if (j < M) {
+            // point row: monomial j at the own point (Neumann/Robin rows: alpha P + beta dP/dn)
+            if (is_pt) {
+                v = mono_op<DIM>(s_mono[j], RBFX_OP_VALUE, 0, 0, xi);
+                if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[j], bn, xi);
             }
-        }
-
-        // ---- stencil classification (types.jl:208-223)
-        bool hermite = false;
-        int bt = 0;  // 0 interior, 1 Dirichlet, 2 Neumann/Robin
-        double ba = 0.0, bb = 0.0, bn[DIM];
+        } else {
+            const int jj = j - M;
+            if (jj < kk) {
+                double xj[DIM];
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) bn[d] = 0.0;
-        if (P.is_boundary) {
-            const bool ib = is_pt && P.is_boundary[g];
-            hermite = __ballot(ib) != 0ull;
-            if (hermite && e < P.n_data && P.is_boundary[e]) {
-                const int bi = P.bnd_index[e];
-                if (P.bc_alpha[bi] == 1.0 && P.bc_beta[bi] == 0.0) {
-                    // identity row (fill_dirichlet_entry!, execution.jl:297-303)
-                    if (L < n_cols) P.out_val[(int64_t)L * P.out_nnz + o0] = 1.0;
-                    if (L == 0) {
-                        if (P.out_col) P.out_col[o0] = (int32_t)e;
-                        if (P.status) P.status[e] = RBFX_ST_DIRICHLET;
+                for (int d = 0; d < DIM; ++d) xj[d] = S.xi[jj][d];
+                const int tj = S.bt[jj];
+                if (is_poly) {
+                    // polynomial row li: monomial li at point jj
+                    v = mono_op<DIM>(s_mono[li], RBFX_OP_VALUE, 0, 0, xj);
+                    if (hermite && tj == 2) {
+                        double nj[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) nj[d] = S.bn[jj][d];
+                        v = S.ba[jj] * v + S.bb[jj] * mono_normal<DIM>(s_mono[li], nj, xj);
                     }
-                    return;
-                }
-            }
-            if (hermite && ib) {
-                const int bi = P.bnd_index[g];
-                ba = P.bc_alpha[bi];
-                bb = P.bc_beta[bi];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) bn[d] = P.normals[(int64_t)bi * DIM + d];
-                bt = (ba == 1.0 && bb == 0.0) ? 1 : 2;
-            }
-        }
-
-        // ---- radius-scaled local frame
-        double h = sqrt(wave_max(r2));
-        if (!(h > 0.0)) h = 1.0;
-        const double hinv = 1.0 / h;
-        double xi[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) xi[d] = D[d] / h;
-        bb = bb / h;  // beta acts through a first derivative
-        if (is_pt) {
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) s_xi[pi][d] = xi[d];
-            if (hermite) {
-                s_bt[pi] = bt;
-                s_ba[pi] = ba;
-                s_bb[pi] = bb;
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) s_bn[pi][d] = bn[d];
-            }
-        }
-        int eval_local = -1;
-        if (hermite) {
-            bool eq = is_pt;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d) eq = eq && (D[d] == 0.0);
-            const unsigned long long m = __ballot(eq);
-            if (!m) {
-                if (L == 0 && P.status) P.status[e] = RBFX_ST_EVAL_NOT_FOUND;
-                __syncthreads();
-                return;
-            }
-            eval_local = uniform(__builtin_ctzll(m)) - M;
-        }
-        __syncthreads();
-
-        BasisDev B;
-        B.kind = P.basis_kind;
-        B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
-
-        // ---- assembly (FP64) into registers
-        T r[NMAX + RC];
-        double a64[REFINE ? NMAX : 1];
-        double diag64 = 0.0;
-#pragma unroll
-        for (int j = 0; j < NMAX + RC; ++j) r[j] = (T)0;
-#pragma unroll
-        for (int j = 0; j < (REFINE ? NMAX : 1); ++j) a64[j] = 0.0;
-        if (is_pt) {
-#pragma unroll
-            for (int a = 0; a < NMAX; ++a) {
-                if (a < M) {
-                    double pv = mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi);
-                    if (hermite && bt == 2) pv = ba * pv + bb * mono_normal<DIM>(s_mono[a], bn, xi);
-                    r[a] = (T)pv;
-                    if constexpr (REFINE) a64[a] = pv;
-                    s_P[pi * M + a] = pv;
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < NMAX; ++j) {
-                if (j >= M && j < n) {
-                    const int jj = j - M;
+                } else if (is_pt) {
                     double Dj[DIM], q2 = 0.0;
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        Dj[d] = xi[d] - s_xi[jj][d];
+                        Dj[d] = xi[d] - xj[d];
                         q2 += Dj[d] * Dj[d];
                     }
-                    double v = phi_r2(B, q2);
-                    if (hermite) {
-                        const int tj = s_bt[jj];
-                        if (bt == 2 || tj == 2) {
-                            double nj[DIM];
+                    v = phi_r2(B, q2);
+                    if (hermite && (bt == 2 || tj == 2)) {
+                        double nj[DIM];
 #pragma unroll
-                            for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
-                            double Di, Djn, D2;
-                            phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
-                            const double aj = s_ba[jj], bj = s_bb[jj];
-                            if (bt != 2) v = aj * v - bj * Djn;
-                            else if (tj != 2) v = ba * v + bb * Di;
-                            else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
-                        }
+                        for (int d = 0; d < DIM; ++d) nj[d] = S.bn[jj][d];
+                        double Di, Djn, D2;
+                        phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
+                        const double aj = S.ba[jj], bj = S.bb[jj];
+                        if (bt != 2) v = aj * v - bj * Djn;
+                        else if (tj != 2) v = ba * v + bb * Di;
+                        else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
                     }
                     if (jj == pi) diag64 = v;
-                    r[j] = (T)v;
-                    if constexpr (REFINE) a64[j] = v;
                 }
             }
         }
-        __syncthreads();
-        if (L < M) {
-#pragma unroll
-            for (int j = 0; j < NMAX; ++j) {
-                if (j >= M && j < n) {
-                    const double v = s_P[(j - M) * M + L];
-                    r[j] = (T)v;
-                    if constexpr (REFINE) a64[j] = v;
-                }
-            }
-        }
+        reg_set<UW, T>(r, j, (T)v);
+        if constexpr (REFINE) reg_set<NMAX, double>(a64, j, v);
+    }
 
-        // ---- right-hand sides (FP64), augmented as register columns NMAX..NMAX+RC-1
-        {
-            double bcol[RC];
+    // ---- right-hand sides (FP64), augmented as register columns NMAX..NMAX+RC-1
+    double b64[REFINE ? RC : 1];
+    {
+        double bcol[RC];
 #pragma unroll
-            for (int c = 0; c < RC; ++c) bcol[c] = 0.0;
-            const bool f7 = hermite && s_bt[eval_local] == 2;
-            if (is_pt) {
-                double De[DIM], q2 = 0.0;
+        for (int c = 0; c < RC; ++c) bcol[c] = 0.0;
+        const bool f7 = alive && hermite && S.bt[eval_local] == 2;
+        if (is_pt) {
+            double De[DIM], q2 = 0.0;
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) {
-                    De[d] = -xi[d];
-                    q2 += De[d] * De[d];
-                }
+            for (int d = 0; d < DIM; ++d) {
+                De[d] = -xi[d];
+                q2 += De[d] * De[d];
+            }
+            for (int t = 0; t < n_terms; ++t) {
+                const TermDev tm = s_terms[t];
+                double v = phi_term<DIM>(B, tm.kind, tm.d1, tm.d2, De, q2);
+                if (hermite && bt == 2) v = ba * v + bb * phi_term_normal<DIM>(B, tm.kind, tm.d1, De, q2, bn);
+                const double sc = term_scale(tm, hinv);
+#pragma unroll
+                for (int c = 0; c < RC; ++c)
+                    if (tm.col == c) bcol[c] += sc * v;
+            }
+        } else if (is_poly) {
+            double z[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) z[d] = 0.0;
+            const int(&ea)[3] = s_mono[li];
+            if (f7) {
+                // the eval node's boundary operator applied to P (assembly.jl:452-467)
+                const double eav = S.ba[eval_local], ebv = S.bb[eval_local];
+                double en[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) en[d] = S.bn[eval_local][d];
+                const double v = eav * mono_op<DIM>(ea, RBFX_OP_VALUE, 0, 0, z) + ebv * mono_normal<DIM>(ea, en, z);
+#pragma unroll
+                for (int c = 0; c < RC; ++c) bcol[c] = v;
+            } else {
                 for (int t = 0; t < n_terms; ++t) {
                     const TermDev tm = s_terms[t];
-                    double v = phi_term<DIM>(B, tm.kind, tm.d1, tm.d2, De, q2);
-                    if (hermite && bt == 2) v = ba * v + bb * phi_term_normal<DIM>(B, tm.kind, tm.d1, De, q2, bn);
-                    const double sc = term_scale(tm, hinv);
+                    const double v = term_scale(tm, hinv) * mono_op<DIM>(ea, tm.kind, tm.d1, tm.d2, z);
 #pragma unroll
                     for (int c = 0; c < RC; ++c)
-                        if (tm.col == c) bcol[c] += sc * v;
+                        if (tm.col == c) bcol[c] += v;
                 }
-            } else if (L < M) {
-                double z[DIM];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) z[d] = 0.0;
-                const int(&ea)[3] = s_mono[L < RBFX_MAX_N ? L : 0];
-                if (f7) {
-                    // the eval node's boundary operator applied to P (assembly.jl:452-467)
-                    const double eav = s_ba[eval_local], ebv = s_bb[eval_local];
-                    double en[DIM];
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) en[d] = s_bn[eval_local][d];
-                    const double v = eav * mono_op<DIM>(ea, RBFX_OP_VALUE, 0, 0, z) + ebv * mono_normal<DIM>(ea, en, z);
-#pragma unroll
-                    for (int c = 0; c < RC; ++c) bcol[c] = v;
-                } else {
-                    for (int t = 0; t < n_terms; ++t) {
-                        const TermDev tm = s_terms[t];
-                        const double v = term_scale(tm, hinv) * mono_op<DIM>(ea, tm.kind, tm.d1, tm.d2, z);
-#pragma unroll
-                        for (int c = 0; c < RC; ++c)
-                            if (tm.col == c) bcol[c] += v;
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < RC; ++c) {
-                r[NMAX + c] = (T)bcol[c];
-                if constexpr (REFINE) s_b64[c][L] = bcol[c];
             }
         }
-        __syncthreads();
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            r[NMAX + c] = (T)bcol[c];
+            if constexpr (REFINE) b64[c] = bcol[c];
+        }
+    }
 
-        // ---- factorisation with rook pivoting (runtime step loop)
-        T diag = (T)diag64;
-        bool act = L < n;
-        int myslot = 1 << 20;
-        unsigned long long amask = __ballot(act);
-        const T alpha = (T)0.6403882032022076;  // (1 + sqrt(17)) / 8
-        bool failed = false;
-        int slot = 0;
-        while (amask != 0ull) {
-            const int c0 = uniform(__builtin_ctzll(amask));
-            // query column c0 (row c0 by symmetry)
-            if (L == c0) {
+    // ---- factorisation with rook pivoting (runtime step loop; every lane takes part)
+    const T alpha = (T)0.6403882032022076;  // (1 + sqrt(17)) / 8
+    T diag = (T)diag64;
+    T Lm[REFINE ? NMAX : 1];
 #pragma unroll
-                for (int j = 0; j < NMAX; ++j) s_col[j] = r[j];
-            }
-            __syncthreads();
-            T v = (act && L != c0) ? (T)fabs(s_col[L & (NMAX - 1)]) : (T)-1;
-            int vi = L;
-            wave_argmax(v, vi);
-            __syncthreads();
-            T colmax = v > (T)0 ? v : (T)0;
-            int imax = uniform(vi);
-            const T absakk = (T)fabs(readlane(diag, c0));
-            if (!(absakk > (T)0 || colmax > (T)0)) { failed = true; break; }
-            int kstep = 1, p = c0, kp = c0;
-            if (absakk < alpha * colmax) {
-                for (int it = 0; it < 2 * NMAX; ++it) {
-                    if (L == imax) {
-#pragma unroll
-                        for (int j = 0; j < NMAX; ++j) s_col[j] = r[j];
-                    }
-                    __syncthreads();
-                    T rv = (act && L != imax) ? (T)fabs(s_col[L & (NMAX - 1)]) : (T)-1;
-                    int ri = L;
-                    wave_argmax(rv, ri);
-                    __syncthreads();
-                    const T rowmax = rv > (T)0 ? rv : (T)0;
-                    const int jmax = uniform(ri);
-                    const T aii = (T)fabs(readlane(diag, imax));
-                    if (!(aii < alpha * rowmax)) { kp = imax; break; }
-                    if (p == jmax || rowmax <= colmax || it == 2 * NMAX - 1) { kp = imax; kstep = 2; break; }
+    for (int j = 0; j < (REFINE ? NMAX : 1); ++j) Lm[j] = (T)0;
+    bool act = alive && li < n;
+    int myslot = 1 << 20;
+    int slot = 0;
+    bool failed = false;
+    for (int step = 0;; ++step) {
+        const unsigned long long gact = group_bits<NMAX>(__ballot(act));
+        if (__ballot(gact != 0ull) == 0ull) break;  // every group of the wave is done
+        const bool busy = gact != 0ull;             // this lane's group still factorises
+        const int c0 = busy ? (int)__builtin_ctzll(gact) : 0;
+        // column c0 = row c0 by symmetry: each lane reads its own entry
+        T v = (T)fabs(group_reg_get<NMAX>(r, c0));
+        if (!(act && li != c0)) v = (T)-1;
+        int vi = li;
+        group_argmax<NMAX>(v, vi);
+        T colmax = v > (T)0 ? v : (T)0;
+        int imax = vi;
+        const T absakk = (T)fabs(group_bcast<NMAX>(diag, c0));
+        int kstep = 1, p = c0, kp = c0;
+        bool need_rook = busy && absakk < alpha * colmax;
+        if (busy && !(absakk > (T)0 || colmax > (T)0)) {
+            failed = true;  // zero column: singular
+            need_rook = false;
+        }
+        // rook iterations: every group takes part in the collective steps
+        for (int it = 0; it < 2 * NMAX; ++it) {
+            if (__ballot(need_rook) == 0ull) break;
+            T rv = (T)fabs(group_reg_get<NMAX>(r, imax));
+            if (!(need_rook && act && li != imax)) rv = (T)-1;
+            int ri = li;
+            group_argmax<NMAX>(rv, ri);
+            const T rowmax = rv > (T)0 ? rv : (T)0;
+            const T aii = (T)fabs(group_bcast<NMAX>(diag, imax));
+            if (need_rook) {
+                if (!(aii < alpha * rowmax)) {
+                    kp = imax;
+                    need_rook = false;
+                } else if (p == ri || rowmax <= colmax || it == 2 * NMAX - 1) {
+                    kp = imax;
+                    kstep = 2;
+                    need_rook = false;
+                } else {
                     p = imax;
                     colmax = rowmax;
-                    imax = jmax;
+                    imax = ri;
                 }
             }
-            if (kstep == 1) {
-                const int pv = kp;
-                if (L == pv) {
-#pragma unroll
-                    for (int j = 0; j < NMAX; ++j) s_U[slot][j] = r[j];
-                }
-                if (L == 0) { s_piv[slot] = pv; s_blk[slot] = 1; }
-                __syncthreads();
-                const T d = readlane(diag, pv);
-                if (d == (T)0) { failed = true; break; }
-                const T dinv = (T)1 / d;
-                T rb[RC];
-#pragma unroll
-                for (int c = 0; c < RC; ++c) rb[c] = (c < n_cols) ? readlane(r[NMAX + c], pv) : (T)0;
-                if (act && L != pv) {
-                    const T a = s_U[slot][L];
-                    const T l = a * dinv;
-                    diag -= l * a;
-#pragma unroll
-                    for (int j = 0; j < NMAX; ++j) r[j] -= l * s_U[slot][j];
-#pragma unroll
-                    for (int c = 0; c < RC; ++c)
-                        if (c < n_cols) r[NMAX + c] -= l * rb[c];
-                    if constexpr (REFINE) s_Lm[slot][L] = l;
-                }
-                if (L == pv) { act = false; myslot = slot; }
-                slot += 1;
-            } else {
-                const int pp = p, qq = kp;
-                if (L == pp) {
-#pragma unroll
-                    for (int j = 0; j < NMAX; ++j) s_U[slot][j] = r[j];
-                }
-                if (L == qq) {
-#pragma unroll
-                    for (int j = 0; j < NMAX; ++j) s_U[slot + 1][j] = r[j];
-                }
-                if (L == 0) {
-                    s_piv[slot] = pp; s_blk[slot] = 2;
-                    s_piv[slot + 1] = qq; s_blk[slot + 1] = 0;
-                }
-                __syncthreads();
-                const T a = readlane(diag, pp), bq = s_U[slot][qq], c = readlane(diag, qq);
-                const T det = a * c - bq * bq;
-                if (det == (T)0) { failed = true; break; }
-                const T idet = (T)1 / det;
-                T rb0[RC], rb1[RC];
-#pragma unroll
-                for (int cc = 0; cc < RC; ++cc) {
-                    rb0[cc] = (cc < n_cols) ? readlane(r[NMAX + cc], pp) : (T)0;
-                    rb1[cc] = (cc < n_cols) ? readlane(r[NMAX + cc], qq) : (T)0;
-                }
-                if (act && L != pp && L != qq) {
-                    const T u0 = s_U[slot][L], u1 = s_U[slot + 1][L];
-                    const T l0 = (u0 * c - u1 * bq) * idet;
-                    const T l1 = (u1 * a - u0 * bq) * idet;
-                    diag -= l0 * u0 + l1 * u1;
-#pragma unroll
-                    for (int j = 0; j < NMAX; ++j) r[j] -= l0 * s_U[slot][j] + l1 * s_U[slot + 1][j];
-#pragma unroll
-                    for (int cc = 0; cc < RC; ++cc)
-                        if (cc < n_cols) r[NMAX + cc] -= l0 * rb0[cc] + l1 * rb1[cc];
-                    if constexpr (REFINE) {
-                        s_Lm[slot][L] = l0;
-                        s_Lm[slot + 1][L] = l1;
-                    }
-                }
-                if (L == pp) { act = false; myslot = slot; }
-                if (L == qq) { act = false; myslot = slot + 1; }
-                slot += 2;
-            }
-            amask = __ballot(act);
-            __syncthreads();
         }
-
-        int st = failed ? RBFX_ST_SINGULAR : P.ok_status;
-        if (!failed) {
-            bool converged = true;
+        // a singular group stops here (failed is uniform over the group)
+        const bool go = busy && !failed;
+        const int p0 = (kstep == 1) ? kp : p;
+        const int p1 = kp;  // equals p0 for a 1x1 pivot
+        const int buf = step & 1;
+        // a wave with a 2x2 pivot in either group runs the two-row update for both;
+        // a 1x1 group then carries its pivot row in both slots with a zero multiplier
+        const bool two_any = __ballot(go && kstep == 2) != 0ull;
+        // pivot row(s) to LDS: the broadcast source of the update
+        if (go && li == p0) {
 #pragma unroll
-            for (int c = 0; c < RC; ++c) {
-                if (c < n_cols) {
-                    const T x = back_substitute<T, NMAX>(s_U, s_piv, s_blk, n, myslot, diag, r[NMAX + c]);
-                    if constexpr (REFINE) {
-                        // FP64 residual b - A x, FP32 correction on the same factors
-                        double x64 = (double)x;
-                        double rel = 1.0;
-                        for (int it = 0; it < P.refine_steps; ++it) {
-                            s_x[L] = x64;
-                            __syncthreads();
-                            double res = s_b64[c][L];
-#pragma unroll
-                            for (int j = 0; j < NMAX; ++j) res -= a64[j] * s_x[j];
-                            __syncthreads();
-                            const T y = forward_replay<T>(s_Lm, s_piv, s_blk, n, myslot, (T)res);
-                            const T dx = back_substitute<T, NMAX>(s_U, s_piv, s_blk, n, myslot, diag, y);
-                            x64 += (double)dx;
-                            const double num = wave_max(L < n ? fabs((double)dx) : 0.0);
-                            const double den = wave_max(L < n ? fabs(x64) : 0.0);
-                            rel = den > 0.0 ? num / den : num;
-                            if (rel <= 0.25 * P.accept_tol) break;
-                        }
-                        if (!(rel <= P.accept_tol)) converged = false;
-                        if (is_pt) P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = x64;
-                    } else {
-                        if (is_pt) P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = (double)x;
-                    }
-                }
-            }
-            if constexpr (REFINE) {
-                st = converged ? RBFX_ST_REFINED : RBFX_ST_FALLBACK64;
-                if (!converged && L == 0 && P.fail_list) {
-                    const int slot2 = atomicAdd(P.fail_count, 1);
-                    P.fail_list[slot2] = (int32_t)e;
-                }
-            }
-        } else {
-            if (REFINE && P.fail_list) {
-                // FP32 pivot breakdown: retry in FP64 before calling the system singular
-                st = RBFX_ST_FALLBACK64;
-                if (L == 0) {
-                    const int slot2 = atomicAdd(P.fail_count, 1);
-                    P.fail_list[slot2] = (int32_t)e;
-                }
-            }
-            for (int c = 0; c < n_cols; ++c)
-                if (is_pt) P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = __builtin_nan("");
+            for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
         }
-        if (is_pt && P.out_col) P.out_col[o0 + pi] = g;
-        if (L == 0 && P.status) P.status[e] = st;
+        if (go && two_any && li == p1) {
+#pragma unroll
+            for (int j = 0; j < UW; ++j) S.prow[buf][1][j] = r[j];
+        }
+        if (go && li == 0) {
+            S.piv[slot] = p0;
+            S.blk[slot] = kstep;
+            if (kstep == 2) {
+                S.piv[slot + 1] = p1;
+                S.blk[slot + 1] = 0;
+            }
+        }
         __syncthreads();
+        const T d0 = group_bcast<NMAX>(diag, p0);
+        const T d1 = group_bcast<NMAX>(diag, p1);
+        const T u0 = S.prow[buf][0][li];
+        T l0, l1 = (T)0;
+        if (kstep == 1) {
+            if (go && d0 == (T)0) failed = true;
+            l0 = u0 / d0;
+        } else {
+            const T bq = S.prow[buf][0][p1];
+            const T det = d0 * d1 - bq * bq;
+            if (go && det == (T)0) failed = true;
+            const T u1 = S.prow[buf][1][li];
+            const T idet = (T)1 / det;
+            l0 = (u0 * d1 - u1 * bq) * idet;
+            l1 = (u1 * d0 - u0 * bq) * idet;
+        }
+        // rows still to be eliminated; every other lane applies zero multipliers, which
+        // leaves its (finite) registers unchanged
+        const bool mine = go && !failed && act && li != p0 && li != p1;
+        if (!mine) {
+            l0 = (T)0;
+            l1 = (T)0;
+        }
+        // in-place fused multiply-adds keep the register row the only live copy
+        const T ml0 = -l0, ml1 = -l1;
+        diag = fma(ml0, u0, diag);
+        axpy_row<NMAX, RC>(r, ml0, S.prow[buf][0], n_cols);
+        if (two_any) {
+            diag = fma(ml1, S.prow[buf][1][li], diag);
+            axpy_row<NMAX, RC>(r, ml1, S.prow[buf][1], n_cols);
+        }
+        if constexpr (REFINE) {
+            group_reg_set<NMAX>(Lm, p0, l0, mine);
+            group_reg_set<NMAX>(Lm, p1, l1, mine && kstep == 2);
+        }
+        if (go && !failed) {
+            if (li == p0) {
+                act = false;
+                myslot = slot;
+            }
+            if (kstep == 2 && li == p1) {
+                act = false;
+                myslot = slot + 1;
+            }
+            slot += kstep;
+        }
+        if (failed) act = false;
     }
+    __syncthreads();
+
+    // a singular group has left all its rows inactive and reports NaN weights
+    const bool gfail = group_bits<NMAX>(__ballot(failed)) != 0ull;
+    int st = gfail ? RBFX_ST_SINGULAR : P.ok_status;
+    const int nslots = gfail ? 0 : slot;
+    int nslots_wave = nslots;
+#pragma unroll
+    for (int m = NMAX; m < 64; m <<= 1) nslots_wave = max(nslots_wave, __shfl_xor(nslots_wave, m, 64));
+    nslots_wave = __builtin_amdgcn_readfirstlane(nslots_wave);
+
+    T y[RC], x[RC];
+#pragma unroll
+    for (int c = 0; c < RC; ++c) y[c] = r[NMAX + c];
+    back_substitute<NMAX, RC, T>(r, S.piv, S.blk, nslots, nslots_wave, myslot, diag, n_cols, y, x);
+    if constexpr (!REFINE) {
+#pragma unroll
+        for (int c = 0; c < RC; ++c)
+            if (c < n_cols && is_pt)
+                P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = gfail ? __builtin_nan("") : (double)x[c];
+    } else {
+        // FP64 residual b - A x, FP32 correction on the same factors, per column
+        bool converged = !gfail;
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            if (c < n_cols) {
+                double x64 = (double)x[c];
+                double rel = 1.0;
+                for (int it = 0; it < P.refine_steps; ++it) {
+                    s_x[g][li] = x64;
+                    __syncthreads();
+                    double res = b64[c];
+#pragma unroll
+                    for (int j = 0; j < NMAX; ++j) res -= a64[j] * s_x[g][j];
+                    __syncthreads();
+                    T yy[1] = {forward_replay<NMAX, T>(Lm, S.piv, S.blk, nslots, nslots_wave, myslot, (T)res)};
+                    T dx[1];
+                    back_substitute<NMAX, 1, T>(r, S.piv, S.blk, nslots, nslots_wave, myslot, diag, 1, yy, dx);
+                    x64 += (double)dx[0];
+                    const double num = group_max<NMAX>(li < n ? fabs((double)dx[0]) : 0.0);
+                    const double den = group_max<NMAX>(li < n ? fabs(x64) : 0.0);
+                    rel = den > 0.0 ? num / den : num;
+                    if (__ballot(!(rel <= 0.25 * P.accept_tol)) == 0ull) break;
+                }
+                if (!(rel <= P.accept_tol)) converged = false;
+                if (is_pt) P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = gfail ? __builtin_nan("") : x64;
+            }
+        }
+        if (alive) {
+            st = converged ? RBFX_ST_REFINED : RBFX_ST_FALLBACK64;
+            if (st == RBFX_ST_FALLBACK64 && li == 0 && P.fail_list) {
+                // refinement did not settle or FP32 pivots broke down: re-solve in FP64
+                const int s2 = atomicAdd(P.fail_count, 1);
+                P.fail_list[s2] = (int32_t)e;
+            }
+        }
+    }
+    if (is_pt && P.out_col) P.out_col[o0 + pi] = gidx;
+    if (alive && li == 0 && P.status) P.status[e] = st;
 }
 
-// ---------------------------------------------------------------- dispatch
-template <typename T, int NMAX, int DIM>
+template <typename T, int NMAX, int DIM, int RC>
 inline hipError_t launch_one(const BuildDev& P, int64_t n_work, hipStream_t s)
 {
     if (n_work <= 0) return hipSuccess;
-    // one wavefront per stencil; a work-list launch sizes the grid by its upper bound
-    const int64_t grid = n_work;
-    hipLaunchKernelGGL((k_build<T, NMAX, DIM>), dim3((unsigned)grid), dim3(64), 0, s, P);
+    constexpr int G = 64 / NMAX;
+    // one group of NMAX lanes per stencil; a work-list launch sizes the grid by its upper bound
+    const int64_t grid = (n_work + G - 1) / G;
+    hipLaunchKernelGGL((k_build<T, NMAX, DIM, RC>), dim3((unsigned)grid), dim3(64), 0, s, P);
     return hipGetLastError();
+}
+
+template <typename T, int NMAX, int DIM>
+inline hipError_t dispatch_rc(const BuildDev& P, int64_t n_work, hipStream_t s)
+{
+    if (P.n_cols <= 1) return launch_one<T, NMAX, DIM, 1>(P, n_work, s);
+    if (P.n_cols <= 4) return launch_one<T, NMAX, DIM, 4>(P, n_work, s);
+    return launch_one<T, NMAX, DIM, 8>(P, n_work, s);
 }
 
 template <typename T, int NMAX>
 inline hipError_t dispatch_dim(const BuildDev& P, int dim, int64_t n_work, hipStream_t s)
 {
     switch (dim) {
-    case 1: return launch_one<T, NMAX, 1>(P, n_work, s);
-    case 2: return launch_one<T, NMAX, 2>(P, n_work, s);
-    case 3: return launch_one<T, NMAX, 3>(P, n_work, s);
+    case 1: return dispatch_rc<T, NMAX, 1>(P, n_work, s);
+    case 2: return dispatch_rc<T, NMAX, 2>(P, n_work, s);
+    case 3: return dispatch_rc<T, NMAX, 3>(P, n_work, s);
     default: return hipErrorInvalidValue;
     }
 }

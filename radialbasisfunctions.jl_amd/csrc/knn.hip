@@ -111,6 +111,7 @@ template <int DIM>
 __device__ __forceinline__ double sqdist(const double* a, const double (&q)[DIM])
 {
     // no contraction: same rounding as sum((x .- y).^2) evaluated term by term
+#pragma clang fp contract(off)
     double s = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
@@ -162,7 +163,7 @@ __global__ __launch_bounds__(128) void k_query(const double* __restrict__ sx, co
             const int y0 = (DIM >= 2) ? max(lo[1], 0) : 0, y1 = (DIM >= 2) ? min(hi[1], G.n[1] - 1) : 0;
             for (int cy = y0; cy <= y1; ++cy) {
                 const bool yedge = (DIM >= 2) && (cy == lo[1] || cy == hi[1]);
-                const bool full_row = zedge || yedge || DIM == 1;
+                const bool full_row = zedge || yedge || (DIM == 1 && rho == 0);
                 const int x0 = max(lo[0], 0), x1 = min(hi[0], G.n[0] - 1);
                 const int step = full_row ? 1 : (hi[0] - lo[0] > 0 ? hi[0] - lo[0] : 1);
                 for (int cx = full_row ? x0 : lo[0]; cx <= (full_row ? x1 : hi[0]); cx += step) {

@@ -487,7 +487,8 @@ class RadialBasisOperator:
                    else np.arange(self.n_eval + 1, dtype=np.int64) * self.k)
             col = self.col.cpu().numpy()
             vals = self.vals.cpu().numpy()
-            mats = tuple(sp.csr_matrix((vals[c], col, ptr), shape=(self.n_eval, self.n_data))
+            # each matrix owns its index arrays: scipy may canonicalise them in place
+            mats = tuple(sp.csr_matrix((vals[c], col.copy(), ptr.copy()), shape=(self.n_eval, self.n_data))
                          for c in range(self.n_cols))
             self._host_weights = mats[0] if (self.rank == 0 and self.n_cols == 1) else mats
         return self._host_weights

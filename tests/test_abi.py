@@ -1,0 +1,75 @@
+"""CPU checks of the C ABI: the library loads and exports every entry point that
+include/rbfx.h declares, and the ctypes mirror has the header's struct sizes.
+No compute calls are made (there may be no GPU)."""
+import ctypes as C
+import pathlib
+import re
+
+ROOT = pathlib.Path(__file__).resolve().parent.parent
+
+
+def declared_symbols():
+    text = (ROOT / "include" / "rbfx.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(rbfx_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_declared_symbols(RBF):
+    lib = C.CDLL(str(ROOT / "radialbasisfunctions.jl_amd" / "librbfx.so"))
+    syms = declared_symbols()
+    assert "rbfx_build_weights" in syms and "rbfx_knn" in syms and "rbfx_spmv" in syms
+    for s in syms:
+        assert hasattr(lib, s), f"librbfx.so does not export {s}"
+    assert set(syms) <= set(RBF._lib.exported_symbols())
+
+
+def test_abi_version_without_gpu():
+    lib = C.CDLL(str(ROOT / "radialbasisfunctions.jl_amd" / "librbfx.so"))
+    lib.rbfx_abi_version.restype = C.c_int32
+    assert lib.rbfx_abi_version() == 1
+
+
+def test_ctypes_mirror_sizes(RBF, O):
+    assert C.sizeof(RBF._abi.BuildArgs) == O.lib().orc_sizeof_build_args()
+    assert C.sizeof(RBF._abi.SpmvArgs) == O.lib().orc_sizeof_spmv_args()
+
+
+def test_operator_lowering(RBF):
+    a = RBF._abi
+    assert RBF.Laplacian().lower(2) == (1, [(0, a.OP_LAP, 0, 0, 1.0)])
+    assert RBF.Partial(2, 2).lower(3) == (1, [(0, a.OP_D2, 1, 0, 1.0)])
+    assert RBF.MixedPartial(1, 1).lower(2) == (1, [(0, a.OP_D2, 0, 0, 1.0)])
+    n, t = (2.0 * RBF.Laplacian() - RBF.Identity()).lower(2)
+    assert n == 1 and sorted(t) == sorted([(0, a.OP_LAP, 0, 0, 2.0), (0, a.OP_VALUE, 0, 0, -1.0)])
+    n, t = RBF.Jacobian(3).lower(3)
+    assert n == 3 and t == [(c, a.OP_D1, c, 0, 1.0) for c in range(3)]
+    n, t = RBF.Hessian(2).lower(2)
+    assert n == 3 and [x[:4] for x in t] == [(0, a.OP_D2, 0, 0), (1, a.OP_MIXED, 0, 1), (2, a.OP_D2, 1, 1)]
+
+
+def test_constructor_errors(RBF):
+    import pytest
+
+    with pytest.raises(RBF.ArgumentError):
+        RBF.PHS(2, poly_deg=-1)
+    with pytest.raises(RBF.ArgumentError):
+        RBF.PHS(3, poly_deg=-2)
+    with pytest.raises(RBF.ArgumentError):
+        RBF.IMQ(-1)
+    with pytest.raises(RBF.ArgumentError):
+        RBF.Gaussian(0.0)
+    with pytest.raises(RBF.ArgumentError):
+        RBF.Partial(3, 1).lower(2)
+    with pytest.raises(RBF.ArgumentError):
+        RBF.Laplacian() + RBF.Jacobian(2)
+    assert repr(RBF.PHS(5, poly_deg=0)) == "Polyharmonic spline (r⁵)\n└─Polynomial augmentation: degree 0"
+    assert RBF.print_op(RBF.Partial(1, 1) * 3.0) == "3.0 × ∂ⁿf/∂xᵢ (n = 1, i = 1)"
+
+
+def test_autoselect_k_goldens(RBF):
+    import json
+
+    g = json.loads((ROOT / "tests" / "golden" / "reference_goldens.json").read_text())["autoselect_k"]
+    pts = g["points"]
+    for c in g["cases"]:
+        assert RBF.autoselect_k(pts[: c["n"]], RBF.PHS(3, poly_deg=c["poly_deg"])) == c["k"], g["src"]

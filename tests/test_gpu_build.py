@@ -1,10 +1,12 @@
 """GPU parity tests of the stencil build, k-NN and apply against the oracle.
 
-Tolerances (FP64 plain method): per stencil ||w_gpu - w_oracle||_inf / ||w_oracle||_inf
-<= 1e-10 against the oracle's centred, radius-scaled arithmetic (the same formulation
-in a different pivot order), and <= 1e-10 for 99.5% of stencils against the
-reference's uncentred arithmetic with every stencil <= 1e-7 (that arithmetic itself
-loses digits on ill-conditioned stencils)."""
+Weight tolerance (FP64 plain method), per stencil, with e(w) = ||w - w*||_inf / ||w*||_inf
+and w* the oracle's solution of the same centred, radius-scaled system in 80-bit
+extended precision:
+    e(w_gpu) <= max(1e-10, 10 * e(w_oracle_fp64))
+i.e. the GPU is within 1e-10 of the exact weights, or at least as accurate (to 10x) as
+the CPU FP64 restatement on stencils whose conditioning puts FP64 itself above 1e-10.
+Against the reference's own uncentred arithmetic the bulk (99.5%) agrees to 1e-10."""
 import json
 import pathlib
 
@@ -18,6 +20,22 @@ GOLD = json.loads((pathlib.Path(__file__).parent / "golden" / "reference_goldens
 def rel_err_rows(wg, wr, k):
     wg, wr = wg.reshape(-1, k), wr.reshape(-1, k)
     return np.abs(wg - wr).max(1) / np.maximum(np.abs(wr).max(1), 1e-300)
+
+
+def check_against_truth(O, wg, pts, evp, adj, basis, op, k, hermite=None):
+    """Apply the module tolerance; returns (e_gpu, e_fp64) per stencil."""
+    _, _, w64, _, rc = O.build(pts, evp, adj, basis, op, scaled=True, hermite=hermite)
+    _, _, wld, _, rc2 = O.build(pts, evp, adj, basis, op, scaled=True, long_double=True, hermite=hermite)
+    assert rc == 0 and rc2 == 0
+    eg, e64 = [], []
+    for c in range(wld.shape[0]):
+        g = rel_err_rows(wg[c], wld[c], k)
+        f = rel_err_rows(w64[c], wld[c], k)
+        bad = g > np.maximum(1e-10, 10 * f)
+        assert not bad.any(), (c, int(bad.sum()), g[bad][:5], f[bad][:5])
+        eg.append(g)
+        e64.append(f)
+    return np.concatenate(eg), np.concatenate(e64)
 
 
 def _op(O, n_cols, terms):
@@ -76,14 +94,13 @@ def test_c1_laplacian_fp64_parity(RBF, O, gpu):
     op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20)
     adj = op.adj_idx.cpu().numpy()
     assert np.array_equal(adj, O.knn(pts, pts, 20))
-    wg = _gpu_weights(op)[0]
+    wg = _gpu_weights(op)
     lap = _op(O, 1, [(0, O.abi.OP_LAP, 0, 0, 1.0)])
     b = O.basis_struct(3, 2)
-    _, _, ws, _, _ = O.build(pts, pts, adj, b, lap, scaled=True)
+    eg, _ = check_against_truth(O, wg, pts, pts, adj, b, lap, 20)
+    assert np.median(eg) <= 1e-12
     _, _, wr, _, _ = O.build(pts, pts, adj, b, lap, scaled=False)
-    es = rel_err_rows(wg, ws[0], 20)
-    er = rel_err_rows(wg, wr[0], 20)
-    assert es.max() <= 1e-10, es.max()
+    er = rel_err_rows(wg[0], wr[0], 20)
     assert np.quantile(er, 0.995) <= 1e-10 and er.max() <= 1e-7, (np.quantile(er, 0.995), er.max())
     # applied operator vs analytic: f = sin4x cos3y, lap f = -25 f
     f = np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1])
@@ -94,8 +111,8 @@ def test_c1_laplacian_fp64_parity(RBF, O, gpu):
     assert nrmse < 0.05
 
 
-BASES = [("PHS1", 1, 1.0), ("PHS3", 3, 1.0), ("PHS5", 5, 1.0), ("PHS7", 7, 1.0), ("IMQ", 10, 3.0),
-         ("Gaussian", 11, 2.0)]
+BASES = [("PHS1", 1, 1.0), ("PHS3", 3, 1.0), ("PHS5", 5, 1.0), ("PHS7", 7, 1.0), ("IMQ", 10, 1.5),
+         ("Gaussian", 11, 1.5)]  # IMQ/Gaussian: eps * stencil radius = 1.5 (well conditioned)
 
 
 def _basis_obj(RBF, name, eps, deg):
@@ -111,17 +128,16 @@ def test_build_parity_bases(RBF, O, gpu, name, kind, eps, dim, deg, k):
         pytest.skip("PHS1 with quadratic augmentation is not a stable pairing")
     rng = np.random.default_rng(kind * 10 + dim * 3 + deg)
     n = 1500
-    pts = rng.random((n, dim)) * (0.25 if kind >= 10 else 1.0)
+    pts = rng.random((n, dim))
+    if kind >= 10:
+        vol = {1: 2.0, 2: np.pi, 3: 4.0 * np.pi / 3.0}[dim]
+        eps = eps / (k / (n * vol)) ** (1.0 / dim)
     basis = _basis_obj(RBF, name, eps, deg)
     L = RBF.Laplacian() + 0.5 * RBF.Partial(1, 1)
     op = RBF.RadialBasisOperator(L, pts, basis=basis, k=k)
     adj = op.adj_idx.cpu().numpy()
-    wg = _gpu_weights(op)[0]
     terms = [(0, O.abi.OP_LAP, 0, 0, 1.0), (0, O.abi.OP_D1, 0, 0, 0.5)]
-    _, _, ws, st, rc = O.build(pts, pts, adj, O.basis_struct(kind, deg, eps), _op(O, 1, terms), scaled=True)
-    assert rc == 0
-    e = rel_err_rows(wg, ws[0], k)
-    assert np.quantile(e, 0.999) <= 1e-10 and e.max() <= 1e-8, (np.quantile(e, 0.999), e.max())
+    check_against_truth(O, _gpu_weights(op), pts, pts, adj, O.basis_struct(kind, deg, eps), _op(O, 1, terms), k)
 
 
 @pytest.mark.parametrize("which", ["d1", "d2", "mixed", "regrid", "grad", "hess3", "sum"])
@@ -154,11 +170,8 @@ def test_build_parity_operators(RBF, O, gpu, which):
     op = RBF.RadialBasisOperator(L, pts, eval_points=ev, basis=b, k=k)
     adj = op.adj_idx.cpu().numpy()
     evp = pts if ev is None else ev
-    _, _, ws, st, rc = O.build(pts, evp, adj, O.basis_struct(b.kind, b.poly_deg), _op(O, nc, terms), scaled=True)
-    wg = _gpu_weights(op)
-    for c in range(nc):
-        e = rel_err_rows(wg[c], ws[c], k)
-        assert e.max() <= 1e-9, (c, np.quantile(e, 0.99), e.max())
+    check_against_truth(O, _gpu_weights(op), pts, evp, adj, O.basis_struct(b.kind, b.poly_deg),
+                        _op(O, nc, terms), k)
 
 
 def test_fp32_refined_matches_fp64(RBF, O, gpu):
@@ -167,8 +180,9 @@ def test_fp32_refined_matches_fp64(RBF, O, gpu):
     o32 = RBF.laplacian(pts, k=20, adjl=o64.adj_idx.cpu().numpy(), precision="fp32_refined")
     counts = o32.status_counts
     assert counts[RBF._abi.ST_REFINED] + counts[RBF._abi.ST_FALLBACK64] == len(pts)
-    e = rel_err_rows(_gpu_weights(o32)[0], _gpu_weights(o64)[0], 20)
-    assert e.max() <= 1e-9, e.max()
+    lap = _op(O, 1, [(0, O.abi.OP_LAP, 0, 0, 1.0)])
+    adj = o64.adj_idx.cpu().numpy()
+    e32, e64 = check_against_truth(O, _gpu_weights(o32), pts, pts, adj, O.basis_struct(3, 2), lap, 20)
 
 
 def test_variable_size_stencils(RBF, O, gpu):
