@@ -118,6 +118,8 @@ typedef struct rbfx_operator {
 /* Arguments of rbfx_build_weights (execution.jl:128-174).
  * Hermite data mirrors BoundaryData (types.jl:178-182) plus the
  * global_to_boundary map (execution.jl:78-93), 0-based, -1 for interior.     */
+struct rbfx_mlp_model;
+
 typedef struct rbfx_build_args {
     int32_t dim;                 /* 1..3                                              */
     int32_t k;                   /* stencil size; the largest one when adj_ptr != NULL */
@@ -146,7 +148,47 @@ typedef struct rbfx_build_args {
     int32_t refine_steps;        /* FP32_REFINED: refinement updates (0 -> 3)         */
     double accept_tol;           /* relative residual tolerance (0 -> default)        */
     void* stream;                /* hipStream_t, NULL = default stream                */
+    /* ABI 2 */
+    double* out_lambda;          /* optional n_cols*n_eval*m: polynomial multipliers
+                                    of the radius-scaled system (the reference keeps
+                                    only the first k rows, assembly.jl:225-226)      */
+    const struct rbfx_mlp_model* model;  /* RBFX_PREC_LEARNED: host struct          */
 } rbfx_build_args;
+
+/* Surrogate for RBFX_PREC_LEARNED (no reference counterpart). A BF16 MLP
+ * (2k -> 256 -> 256 -> 256 -> k+m, GELU, FP32 accumulation) maps the canonical,
+ * radius-scaled stencil offsets to (w~, lambda) of the scaled system. The host
+ * struct holds device pointers; the signature fields must match the build.     */
+typedef struct rbfx_mlp_model {
+    int32_t dim;                 /* 2                                                  */
+    int32_t k;                   /* stencil size the network was trained for (20)      */
+    int32_t basis_kind;          /* RBFX_PHS3                                          */
+    int32_t poly_deg;            /* 2                                                  */
+    int32_t op_kind;             /* RBFX_OP_LAP (rotation-invariant operators only)    */
+    int32_t n_out;               /* k + m                                              */
+    const float* params;         /* rbfx_mlp_param_count() FP32 master parameters      */
+    const uint16_t* wbf16;       /* rbfx_mlp_bf16_count() BF16 weights                 */
+    const double* out_mean;      /* n_out: output de-standardisation y = mean + std*z  */
+    const double* out_std;
+    double accept_tol;           /* accept when max|collocation residual| <=
+                                    accept_tol * max|L phi| (FP64, per stencil)        */
+    double* out_rho;             /* optional n_eval: the relative residual per stencil */
+} rbfx_mlp_model;
+
+/* One forward (and, with target != NULL, backward) pass of the surrogate on a batch. */
+typedef struct rbfx_mlp_run_args {
+    int32_t batch;               /* M stencils                                         */
+    int32_t n_out;               /* outputs used by the loss (k + m <= 32)             */
+    const uint16_t* x;           /* M x 64 BF16 inputs (canonical offsets, zero pad)   */
+    const float* target;         /* M x n_out standardised targets, or NULL            */
+    const float* params;
+    const uint16_t* wbf16;
+    float* grads;                /* rbfx_mlp_param_count() FP32 (with target)           */
+    float* loss;                 /* 1 FP32: batch mean squared error (with target)      */
+    float* y_out;                /* optional M x 32 FP32 network outputs                */
+    void* workspace;             /* rbfx_mlp_workspace_bytes(batch) bytes               */
+    void* stream;
+} rbfx_mlp_run_args;
 
 /* Arguments of rbfx_spmv: Y_v = alpha * W_v X + beta * Y_v for every value array v.
  * W_v share row pointers and column indices (one stencil set, several operators).  */
@@ -202,6 +244,32 @@ int rbfx_spmv(const rbfx_spmv_args* args);
 /* Number of stencils per status code from the last rbfx_build_weights on this
  * thread (counts[0..15]); lets the host raise the reference's exception types. */
 int rbfx_last_build_counts(int64_t counts[16]);
+
+/* ---- surrogate training and inputs (no reference counterpart) ------------ */
+
+/* Sizes of the flat FP32 parameter array, its BF16 weight copy, and the run
+ * workspace for a batch of M stencils.                                          */
+int64_t rbfx_mlp_param_count(void);
+int64_t rbfx_mlp_bf16_count(void);
+int64_t rbfx_mlp_workspace_bytes(int32_t batch);
+
+/* Forward pass; with a target also the mean-squared-error loss and the gradient
+ * of every parameter (BF16 operands, FP32 accumulation on the matrix cores).    */
+int rbfx_mlp_run(const rbfx_mlp_run_args* args);
+
+/* Adam step on the FP32 parameters, then refresh of the BF16 weight copy.
+ * step is the 1-based update count (bias correction).                           */
+int rbfx_mlp_adam(float* params, const float* grads, float* m, float* v, uint16_t* wbf16, float lr, float beta1,
+                  float beta2, float eps, int32_t step, void* stream);
+
+/* BF16 weight copy of a parameter array (after initialising or loading it).    */
+int rbfx_mlp_sync_bf16(const float* params, uint16_t* wbf16, void* stream);
+
+/* Canonical radius-scaled offsets of fixed-size 2-D stencils: out_xi (n_eval*k*2
+ * FP64, may be NULL), the network inputs out_x (n_eval*64 BF16, may be NULL) and
+ * the stencil radius out_h (n_eval, may be NULL).                                */
+int rbfx_stencil_features(const double* data, const double* eval, int64_t n_eval, int32_t dim, int32_t k,
+                          const int32_t* adj_idx, double* out_xi, uint16_t* out_x, double* out_h, void* stream);
 
 #ifdef __cplusplus
 }

@@ -64,7 +64,8 @@ def lib():
                                          C.c_void_p, C.c_void_p, C.POINTER(C.c_int64)]
         L.orc_spmv.argtypes = [C.c_int64, C.POINTER(C.c_int64), C.POINTER(C.c_int32), dp, C.c_int,
                                dp, dp, C.c_int]
-        for f in ("orc_sizeof_build_args", "orc_sizeof_spmv_args", "orc_sizeof_operator"):
+        for f in ("orc_sizeof_build_args", "orc_sizeof_spmv_args", "orc_sizeof_operator", "orc_sizeof_mlp_model",
+                  "orc_sizeof_mlp_run_args"):
             getattr(L, f).restype = C.c_int64
         _LIB = L
     return _LIB

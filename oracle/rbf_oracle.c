@@ -949,3 +949,5 @@ void orc_spmv(int64_t n_rows, const int64_t* row_ptr, const int32_t* col, const 
 int64_t orc_sizeof_build_args(void) { return (int64_t)sizeof(rbfx_build_args); }
 int64_t orc_sizeof_spmv_args(void) { return (int64_t)sizeof(rbfx_spmv_args); }
 int64_t orc_sizeof_operator(void) { return (int64_t)sizeof(rbfx_operator); }
+int64_t orc_sizeof_mlp_model(void) { return (int64_t)sizeof(rbfx_mlp_model); }
+int64_t orc_sizeof_mlp_run_args(void) { return (int64_t)sizeof(rbfx_mlp_run_args); }

@@ -6,6 +6,7 @@ This package is the host-side mirror of the reference's operator layer.
 """
 from . import _abi
 from . import _lib
+from . import learned
 from .basis import (PHS, PHS1, PHS3, PHS5, PHS7, IMQ, Gaussian, MonomialBasis, AbstractRadialBasis, AbstractPHS,
                     BoundaryCondition, Dirichlet, Neumann, Robin, Internal, is_dirichlet, is_neumann, is_robin,
                     is_internal, check_poly_deg, n_monomials)
@@ -14,5 +15,6 @@ from .operators import (AbstractOperator, RadialBasisOperator, Laplacian, Partia
                         Regrid, Identity, ScaledOperator, SumOperator, Custom, Hermite, laplacian, partial,
                         mixed_partial, jacobian, gradient, hessian, regrid, custom, find_neighbors, autoselect_k,
                         weights, update_weights, is_cache_valid, print_op)
+from .learned import LearnedModel
 
 __all__ = [n for n in dir() if not n.startswith("_")]

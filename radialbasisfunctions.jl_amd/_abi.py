@@ -46,6 +46,39 @@ ST_TOO_LARGE = 10
 MAX_TERMS = 16
 MAX_COLS = 8
 MAX_N = 64
+ABI_VERSION = 2
+
+# surrogate network shape (must match csrc/mlp.h)
+MLP_LAYERS = 4
+MLP_IN_PAD = 64
+MLP_HID = 256
+MLP_OUT_PAD = 32
+
+
+def mlp_k(l):
+    return MLP_IN_PAD if l == 0 else MLP_HID
+
+
+def mlp_n(l):
+    return MLP_OUT_PAD if l == MLP_LAYERS - 1 else MLP_HID
+
+
+def mlp_w_off(l):
+    """offset of Wt_l [N_l x K_l] in the flat FP32 parameter array (b_l follows it)"""
+    return sum(mlp_n(i) * mlp_k(i) + mlp_n(i) for i in range(l))
+
+
+def mlp_b_off(l):
+    return mlp_w_off(l) + mlp_n(l) * mlp_k(l)
+
+
+def mlp_wb_off(l):
+    """offset of Wt_l in the BF16 weight copy"""
+    return sum(mlp_n(i) * mlp_k(i) for i in range(l))
+
+
+MLP_N_PARAMS = mlp_w_off(MLP_LAYERS)
+MLP_N_WBF16 = mlp_wb_off(MLP_LAYERS)
 
 
 class Basis(C.Structure):
@@ -92,6 +125,41 @@ class BuildArgs(C.Structure):
         ("precision", C.c_int32),
         ("refine_steps", C.c_int32),
         ("accept_tol", C.c_double),
+        ("stream", C.c_void_p),
+        ("out_lambda", C.c_void_p),
+        ("model", C.c_void_p),
+    ]
+
+
+class MlpModel(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("k", C.c_int32),
+        ("basis_kind", C.c_int32),
+        ("poly_deg", C.c_int32),
+        ("op_kind", C.c_int32),
+        ("n_out", C.c_int32),
+        ("params", C.c_void_p),
+        ("wbf16", C.c_void_p),
+        ("out_mean", C.c_void_p),
+        ("out_std", C.c_void_p),
+        ("accept_tol", C.c_double),
+        ("out_rho", C.c_void_p),
+    ]
+
+
+class MlpRunArgs(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int32),
+        ("n_out", C.c_int32),
+        ("x", C.c_void_p),
+        ("target", C.c_void_p),
+        ("params", C.c_void_p),
+        ("wbf16", C.c_void_p),
+        ("grads", C.c_void_p),
+        ("loss", C.c_void_p),
+        ("y_out", C.c_void_p),
+        ("workspace", C.c_void_p),
         ("stream", C.c_void_p),
     ]
 

@@ -52,13 +52,29 @@ def lib():
                 L.rbfx_spmv.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_last_build_counts.restype = C.c_int
                 L.rbfx_last_build_counts.argtypes = [C.POINTER(C.c_int64)]
+                L.rbfx_mlp_param_count.restype = C.c_int64
+                L.rbfx_mlp_bf16_count.restype = C.c_int64
+                L.rbfx_mlp_workspace_bytes.restype = C.c_int64
+                L.rbfx_mlp_workspace_bytes.argtypes = [C.c_int32]
+                L.rbfx_mlp_run.restype = C.c_int
+                L.rbfx_mlp_run.argtypes = [C.POINTER(abi.MlpRunArgs)]
+                L.rbfx_mlp_adam.restype = C.c_int
+                L.rbfx_mlp_adam.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                            C.c_float, C.c_float, C.c_float, C.c_int32, C.c_void_p]
+                L.rbfx_mlp_sync_bf16.restype = C.c_int
+                L.rbfx_mlp_sync_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+                L.rbfx_stencil_features.restype = C.c_int
+                L.rbfx_stencil_features.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
+                                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
                 _LIB = L
     return _LIB
 
 
 def exported_symbols():
     return ["rbfx_init", "rbfx_last_error", "rbfx_abi_version", "rbfx_knn", "rbfx_build_weights",
-            "rbfx_spmv", "rbfx_last_build_counts"]
+            "rbfx_spmv", "rbfx_last_build_counts", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
+            "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
+            "rbfx_stencil_features"]
 
 
 def _check(rc):
@@ -130,3 +146,56 @@ def spmv(args, dev):
 
 def last_error():
     return lib().rbfx_last_error().decode(errors="replace")
+
+
+def mlp_run(x, params, wbf16, workspace, batch, target=None, grads=None, loss=None, y_out=None, n_out=0):
+    """One forward (and with target, backward) pass of the surrogate (device tensors)."""
+    t = torch()
+    dev = x.device
+    a = abi.MlpRunArgs()
+    a.batch = int(batch)
+    a.n_out = int(n_out)
+    a.x = x.data_ptr()
+    a.target = None if target is None else target.data_ptr()
+    a.params = params.data_ptr()
+    a.wbf16 = wbf16.data_ptr()
+    a.grads = None if grads is None else grads.data_ptr()
+    a.loss = None if loss is None else loss.data_ptr()
+    a.y_out = None if y_out is None else y_out.data_ptr()
+    a.workspace = workspace.data_ptr()
+    a.stream = stream_ptr(dev).value
+    with t.cuda.device(dev):
+        _check(lib().rbfx_mlp_run(C.byref(a)))
+
+
+def mlp_adam(params, grads, m, v, wbf16, lr, beta1, beta2, eps, step):
+    t = torch()
+    dev = params.device
+    with t.cuda.device(dev):
+        _check(lib().rbfx_mlp_adam(ptr(params), ptr(grads), ptr(m), ptr(v), ptr(wbf16), float(lr), float(beta1),
+                                   float(beta2), float(eps), int(step), stream_ptr(dev)))
+
+
+def mlp_sync_bf16(params, wbf16):
+    t = torch()
+    dev = params.device
+    with t.cuda.device(dev):
+        _check(lib().rbfx_mlp_sync_bf16(ptr(params), ptr(wbf16), stream_ptr(dev)))
+
+
+def mlp_workspace_bytes(batch):
+    return int(lib().rbfx_mlp_workspace_bytes(int(batch)))
+
+
+def stencil_features(data, eval_points, adj, k, want_xi=True, want_x=True, want_h=True):
+    """Canonical radius-scaled offsets of fixed-size 2-D stencils (device tensors)."""
+    t = torch()
+    dev = data.device
+    n = eval_points.shape[0]
+    xi = t.empty((n, k, 2), dtype=t.float64, device=dev) if want_xi else None
+    x = t.empty((n, abi.MLP_IN_PAD), dtype=t.bfloat16, device=dev) if want_x else None
+    h = t.empty(n, dtype=t.float64, device=dev) if want_h else None
+    with t.cuda.device(dev):
+        _check(lib().rbfx_stencil_features(ptr(data), ptr(eval_points), n, data.shape[1], int(k), ptr(adj), ptr(xi),
+                                           ptr(x), ptr(h), stream_ptr(dev)))
+    return xi, x, h

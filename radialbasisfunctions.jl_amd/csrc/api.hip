@@ -16,6 +16,8 @@
 
 #include "build.h"
 #include "knn.h"
+#include "learned.h"
+#include "mlp.h"
 #include "rbfx.h"
 #include "spmv.h"
 
@@ -50,6 +52,9 @@ struct DeviceState {
     int32_t* status = nullptr;
     size_t cap_status = 0;
     unsigned long long* hist = nullptr;
+    int32_t* reject_list = nullptr;
+    size_t cap_reject = 0;
+    int32_t* reject_count = nullptr;
 };
 
 std::mutex g_mu;
@@ -101,11 +106,19 @@ int n_mono_host(int dim, int deg)
 
 bool is_phs(int k) { return k == RBFX_PHS1 || k == RBFX_PHS3 || k == RBFX_PHS5 || k == RBFX_PHS7; }
 
+// a device counter read back to size the next launch (blocking)
+hipError_t read_count(const int32_t* d, int32_t* h, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(h, d, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+}
+
 }  // namespace
 
 extern "C" {
 
-int32_t rbfx_abi_version(void) { return 1; }
+int32_t rbfx_abi_version(void) { return 2; }
 
 const char* rbfx_last_error(void) { return g_err.c_str(); }
 
@@ -179,8 +192,26 @@ int rbfx_build_weights(const rbfx_build_args* a)
     if (a->k + M > RBFX_MAX_N)
         return set_err(RBFX_EUNSUPPORTED, "stencil system k+m=%d exceeds the device maximum %d", a->k + M, RBFX_MAX_N);
     if (a->n_eval == 0) return RBFX_OK;
-    if (a->precision < RBFX_PREC_FP64 || a->precision > RBFX_PREC_FP32_REFINED)
+    if (a->precision < RBFX_PREC_FP64 || a->precision > RBFX_PREC_LEARNED)
         return set_err(RBFX_EUNSUPPORTED, "precision mode %d is not available through rbfx_build_weights", a->precision);
+    const rbfx_mlp_model* mdl = a->model;
+    if (a->precision == RBFX_PREC_LEARNED) {
+        // the surrogate covers the configuration it was trained on
+        if (!mdl) return set_err(RBFX_EINVAL, "learned build needs a model");
+        if (!mdl->params || !mdl->wbf16 || !mdl->out_mean || !mdl->out_std)
+            return set_err(RBFX_EINVAL, "learned build: model pointers missing");
+        if (a->dim != 2 || mdl->dim != 2) return set_err(RBFX_EUNSUPPORTED, "learned build supports dim 2 only");
+        if (a->adj_ptr || a->out_ptr || a->k != mdl->k || mdl->k != 20)
+            return set_err(RBFX_EUNSUPPORTED, "learned build needs fixed stencils of the trained size k=%d", mdl->k);
+        if (a->basis.kind != mdl->basis_kind || a->basis.poly_deg != mdl->poly_deg || mdl->poly_deg != 2)
+            return set_err(RBFX_EUNSUPPORTED, "learned build: basis differs from the model's");
+        if (a->is_boundary) return set_err(RBFX_EUNSUPPORTED, "learned build: Hermite stencils are not covered");
+        if (a->op.n_cols != 1) return set_err(RBFX_EUNSUPPORTED, "learned build: one operator column only");
+        for (int t = 0; t < a->op.n_terms; ++t)
+            if (a->op.terms[t].kind != mdl->op_kind || mdl->op_kind != RBFX_OP_LAP)
+                return set_err(RBFX_EUNSUPPORTED, "learned build: only the Laplacian the model was trained on");
+        if (mdl->n_out != a->k + n_mono_host(2, 2)) return set_err(RBFX_EINVAL, "learned build: model n_out mismatch");
+    }
     hipStream_t st = (hipStream_t)a->stream;
 
     // Hermite: the normal-form guard of _check_normal_form_support (api.jl:382-397)
@@ -248,6 +279,7 @@ int rbfx_build_weights(const rbfx_build_args* a)
     P.out_val = a->out_val;
     P.out_nnz = a->out_nnz > 0 ? a->out_nnz : a->n_eval * (int64_t)a->k;
     P.status = status;
+    P.out_lambda = a->out_lambda;
     P.refine_steps = a->refine_steps > 0 ? a->refine_steps : 4;
     P.accept_tol = a->accept_tol > 0 ? a->accept_tol : 1e-12;
     P.ok_status = RBFX_ST_OK;
@@ -261,15 +293,55 @@ int rbfx_build_weights(const rbfx_build_args* a)
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
         P.fail_list = ds.fail_list;
         P.fail_count = ds.fail_count;
-        HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, a->n_eval, st));
+        int64_t n_solve = a->n_eval;
+        if (a->precision == RBFX_PREC_LEARNED) {
+            // surrogate + FP64 acceptance; the rejected stencils go to the solver
+            HIPCHK(grow((void**)&ds.reject_list, ds.cap_reject, sizeof(int32_t) * (size_t)a->n_eval));
+            if (!ds.reject_count) HIPCHK(hipMalloc((void**)&ds.reject_count, sizeof(int32_t)));
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.reject_count);
+            rbfx::LearnedDev Q;
+            std::memset(&Q, 0, sizeof(Q));
+            Q.n_eval = a->n_eval;
+            Q.data = a->data;
+            Q.eval = a->eval;
+            Q.adj_idx = a->adj_idx;
+            Q.k = a->k;
+            Q.basis_kind = kind;
+            Q.eps = a->basis.eps;
+            Q.n_mono = P.n_mono;
+            std::memcpy(Q.mono_e, P.mono_e, sizeof(Q.mono_e));
+            std::memcpy(Q.terms, P.terms, sizeof(Q.terms));
+            Q.n_terms = P.n_terms;
+            Q.op_scale_order = 2.0;
+            Q.wbf16 = reinterpret_cast<const __bf16*>(mdl->wbf16);
+            Q.params = mdl->params;
+            Q.out_mean = mdl->out_mean;
+            Q.out_std = mdl->out_std;
+            Q.accept_tol = mdl->accept_tol;
+            Q.out_col = a->out_col;
+            Q.out_val = a->out_val;
+            Q.status = status;
+            Q.reject_list = ds.reject_list;
+            Q.reject_count = ds.reject_count;
+            Q.out_rho = mdl->out_rho;
+            HIPCHK(rbfx::launch_learned_build(Q, st));
+            int32_t nrej = 0;
+            HIPCHK(read_count(ds.reject_count, &nrej, st));
+            P.list = ds.reject_list;
+            P.list_len = ds.reject_count;
+            n_solve = nrej;
+        }
+        if (n_solve > 0) HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, n_solve, st));
         // FP64 re-solve of the stencils whose refinement did not converge
+        int32_t nfail = 0;
+        HIPCHK(read_count(ds.fail_count, &nfail, st));
         rbfx::BuildDev F = P;
         F.fail_list = nullptr;
         F.fail_count = nullptr;
         F.list = ds.fail_list;
         F.list_len = ds.fail_count;
         F.ok_status = RBFX_ST_FALLBACK64;
-        HIPCHK(rbfx::launch_build(F, a->dim, nmax, false, a->n_eval, st));
+        if (nfail > 0) HIPCHK(rbfx::launch_build(F, a->dim, nmax, false, nfail, st));
     }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ds.hist, 16);
     const int64_t hb = std::min<int64_t>((a->n_eval + 255) / 256, 4096);
@@ -286,6 +358,175 @@ int rbfx_build_weights(const rbfx_build_args* a)
     if (h[RBFX_ST_SINGULAR])
         return set_err(RBFX_ESINGULAR, "%llu stencil systems are singular", h[RBFX_ST_SINGULAR]);
     g_err.clear();
+    return RBFX_OK;
+}
+
+int64_t rbfx_mlp_param_count(void) { return rbfx::mlp_n_params(); }
+
+int64_t rbfx_mlp_bf16_count(void) { return rbfx::mlp_n_wbf16(); }
+
+}  // extern "C"
+
+namespace {
+
+// run workspace: Z_l (FP32), H_l (BF16), dY and two dZ buffers (BF16), 256-B aligned
+struct MlpWs {
+    float* Z[rbfx::kMlpLayers];
+    __bf16* H[rbfx::kMlpLayers - 1];
+    __bf16* dY;
+    __bf16* dZ[2];
+    int64_t bytes;
+};
+
+MlpWs mlp_ws(void* base, int64_t M)
+{
+    MlpWs w;
+    char* p = static_cast<char*>(base);
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) {
+        char* q = p ? p + off : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return q;
+    };
+    for (int l = 0; l < rbfx::kMlpLayers; ++l) w.Z[l] = reinterpret_cast<float*>(take(M * rbfx::mlp_n(l) * 4));
+    for (int l = 0; l < rbfx::kMlpLayers - 1; ++l) w.H[l] = reinterpret_cast<__bf16*>(take(M * rbfx::kHid * 2));
+    w.dY = reinterpret_cast<__bf16*>(take(M * rbfx::kOutPad * 2));
+    w.dZ[0] = reinterpret_cast<__bf16*>(take(M * rbfx::kHid * 2));
+    w.dZ[1] = reinterpret_cast<__bf16*>(take(M * rbfx::kHid * 2));
+    w.bytes = off;
+    return w;
+}
+
+}  // namespace
+
+extern "C" {
+
+int64_t rbfx_mlp_workspace_bytes(int32_t batch) { return batch > 0 ? mlp_ws(nullptr, batch).bytes : 0; }
+
+int rbfx_mlp_run(const rbfx_mlp_run_args* a)
+{
+    using namespace rbfx;
+    if (!a) return set_err(RBFX_EINVAL, "rbfx_mlp_run: null args");
+    if (a->batch < 1) return set_err(RBFX_EINVAL, "rbfx_mlp_run: batch must be >= 1");
+    if (!a->x || !a->params || !a->wbf16 || !a->workspace)
+        return set_err(RBFX_EINVAL, "rbfx_mlp_run: x, params, wbf16 and workspace are required");
+    if (a->target && (!a->grads || !a->loss || a->n_out < 1 || a->n_out > kOutPad))
+        return set_err(RBFX_EINVAL, "rbfx_mlp_run: training needs grads, loss and 1 <= n_out <= %d", kOutPad);
+    hipStream_t st = (hipStream_t)a->stream;
+    const int M = a->batch;
+    MlpWs w = mlp_ws(a->workspace, M);
+    const __bf16* X = reinterpret_cast<const __bf16*>(a->x);
+    const __bf16* WB = reinterpret_cast<const __bf16*>(a->wbf16);
+    float* Zout = a->y_out ? a->y_out : w.Z[kMlpLayers - 1];
+    for (int l = 0; l < kMlpLayers; ++l) {
+        GemmArgs g;
+        std::memset(&g, 0, sizeof(g));
+        g.M = M;
+        g.N = mlp_n(l);
+        g.K = mlp_k(l);
+        g.A = l == 0 ? X : w.H[l - 1];
+        g.sAm = mlp_k(l);
+        g.sAk = 1;
+        g.B = WB + mlp_wb_off(l);
+        g.sBn = mlp_k(l);
+        g.sBk = 1;
+        g.C = l == kMlpLayers - 1 ? Zout : w.Z[l];
+        g.ldc = mlp_n(l);
+        g.bias = a->params + mlp_b_off(l);
+        if (l < kMlpLayers - 1) {
+            g.epi = EPI_BIAS_GELU;
+            g.H = w.H[l];
+            g.ldh = kHid;
+        } else {
+            g.epi = EPI_BIAS;
+        }
+        HIPCHK(launch_gemm(g, st));
+    }
+    if (a->target) {
+        HIPCHK(launch_fill_f32(a->loss, 1, 0.0f, st));
+        HIPCHK(launch_fill_f32(a->grads, mlp_n_params(), 0.0f, st));
+        HIPCHK(launch_mse_grad(M, a->n_out, Zout, kOutPad, a->target, a->n_out, w.dY, kOutPad, a->loss, st));
+        const __bf16* dZ = w.dY;
+        int64_t ld_dz = kOutPad;
+        for (int l = kMlpLayers - 1; l >= 0; --l) {
+            const __bf16* Hin = l == 0 ? X : w.H[l - 1];
+            GemmArgs gw;
+            std::memset(&gw, 0, sizeof(gw));
+            gw.M = mlp_n(l);
+            gw.N = mlp_k(l);
+            gw.K = M;
+            gw.A = dZ;
+            gw.sAm = 1;
+            gw.sAk = ld_dz;
+            gw.B = Hin;
+            gw.sBn = 1;
+            gw.sBk = mlp_k(l);
+            gw.C = a->grads + mlp_w_off(l);
+            gw.ldc = mlp_k(l);
+            gw.epi = EPI_F32;
+            gw.accumulate = 1;
+            gw.k_split = 512;
+            HIPCHK(launch_gemm(gw, st));
+            HIPCHK(launch_colsum_bf16(M, mlp_n(l), dZ, ld_dz, a->grads + mlp_b_off(l), st));
+            if (l > 0) {
+                __bf16* dZn = w.dZ[l & 1];
+                GemmArgs gd;
+                std::memset(&gd, 0, sizeof(gd));
+                gd.M = M;
+                gd.N = mlp_k(l);
+                gd.K = mlp_n(l);
+                gd.A = dZ;
+                gd.sAm = ld_dz;
+                gd.sAk = 1;
+                gd.B = WB + mlp_wb_off(l);
+                gd.sBn = 1;
+                gd.sBk = mlp_k(l);
+                gd.epi = EPI_GELU_BWD;
+                gd.Z = w.Z[l - 1];
+                gd.ldz = kHid;
+                gd.H = dZn;
+                gd.ldh = kHid;
+                HIPCHK(launch_gemm(gd, st));
+                dZ = dZn;
+                ld_dz = kHid;
+            }
+        }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+int rbfx_mlp_adam(float* params, const float* grads, float* m, float* v, uint16_t* wbf16, float lr, float beta1,
+                  float beta2, float eps, int32_t step, void* stream)
+{
+    if (!params || !grads || !m || !v || !wbf16) return set_err(RBFX_EINVAL, "rbfx_mlp_adam: null pointer argument");
+    if (step < 1) return set_err(RBFX_EINVAL, "rbfx_mlp_adam: step must be >= 1");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(rbfx::launch_adam(rbfx::mlp_n_params(), params, grads, m, v, lr, beta1, beta2, eps, step, st));
+    HIPCHK(rbfx::launch_params_to_bf16(params, reinterpret_cast<__bf16*>(wbf16), st));
+    HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+int rbfx_mlp_sync_bf16(const float* params, uint16_t* wbf16, void* stream)
+{
+    if (!params || !wbf16) return set_err(RBFX_EINVAL, "rbfx_mlp_sync_bf16: null pointer argument");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(rbfx::launch_params_to_bf16(params, reinterpret_cast<__bf16*>(wbf16), st));
+    HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+int rbfx_stencil_features(const double* data, const double* eval, int64_t n_eval, int32_t dim, int32_t k,
+                          const int32_t* adj_idx, double* out_xi, uint16_t* out_x, double* out_h, void* stream)
+{
+    if (!data || !eval || !adj_idx) return set_err(RBFX_EINVAL, "rbfx_stencil_features: null pointer argument");
+    if (dim != 2) return set_err(RBFX_EUNSUPPORTED, "rbfx_stencil_features: dim 2 only");
+    if (k < 1 || 2 * k > rbfx::kInPad) return set_err(RBFX_EUNSUPPORTED, "rbfx_stencil_features: need 1 <= k <= 32");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(rbfx::launch_stencil_features(data, eval, n_eval, k, adj_idx, out_xi, reinterpret_cast<__bf16*>(out_x),
+                                         out_h, st));
+    HIPCHK(hipStreamSynchronize(st));
     return RBFX_OK;
 }
 

@@ -36,6 +36,7 @@ struct BuildDev {
     int32_t* out_col;
     double* out_val;
     int64_t out_nnz;
+    double* out_lambda;  // optional: n_cols * n_eval * n_mono multipliers
     int32_t* status;
     // optional work list (stencil ids) whose length lives in device memory
     const int32_t* list;

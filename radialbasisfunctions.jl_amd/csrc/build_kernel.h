@@ -38,6 +38,7 @@
 
 #include "build.h"
 #include "device_math.h"
+#include "mono.h"
 
 namespace rbfx {
 
@@ -206,59 +207,6 @@ __device__ __forceinline__ void group_reg_set(T (&r)[N], int idx, T v, bool pred
         reg_set_if<N, T>(r, a, v, pred && lane_id() < 32);
         reg_set_if<N, T>(r, b, v, pred && lane_id() >= 32);
     }
-}
-
-// ---------------------------------------------------------------- monomials
-// x^p for a runtime exponent 0 <= p <= kMaxPolyDeg, by repeated multiplication
-__device__ __forceinline__ double ipow(double x, int p)
-{
-    double v = 1.0;
-#pragma unroll
-    for (int q = 0; q < kMaxPolyDeg; ++q)
-        if (q < p) v *= x;
-    return v;
-}
-
-// operator term `kind` applied to the monomial x^e
-template <int DIM>
-__device__ __forceinline__ double mono_op(const int (&e)[3], int kind, int d1, int d2, const double (&x)[DIM])
-{
-    if (kind == RBFX_OP_MIXED && d1 == d2) kind = RBFX_OP_D2;
-    if (kind == RBFX_OP_LAP) {
-        double acc = 0.0;
-#pragma unroll
-        for (int dd = 0; dd < DIM; ++dd) {
-            if (e[dd] >= 2) {
-                double t = (double)(e[dd] * (e[dd] - 1));
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) t *= ipow(x[d], d == dd ? e[d] - 2 : e[d]);
-                acc += t;
-            }
-        }
-        return acc;
-    }
-    double coef = 1.0, val = 1.0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) {
-        int o = 0;
-        if (kind == RBFX_OP_D1 && d == d1) o = 1;
-        if (kind == RBFX_OP_D2 && d == d1) o = 2;
-        if (kind == RBFX_OP_MIXED && (d == d1 || d == d2)) o = 1;
-        if (o > e[d]) return 0.0;
-        if (o == 1) coef *= (double)e[d];
-        if (o == 2) coef *= (double)(e[d] * (e[d] - 1));
-        val *= ipow(x[d], e[d] - o);
-    }
-    return coef * val;
-}
-
-template <int DIM>
-__device__ __forceinline__ double mono_normal(const int (&e)[3], const double (&n)[DIM], const double (&x)[DIM])
-{
-    double acc = 0.0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) acc += n[d] * mono_op<DIM>(e, RBFX_OP_D1, d, 0, x);
-    return acc;
 }
 
 // term scale h^-order in the radius-scaled frame
@@ -758,9 +706,12 @@ k_build(const BuildDev P)
     back_substitute<NMAX, RC, T>(r, S.piv, S.blk, nslots, nslots_wave, myslot, diag, n_cols, y, x);
     if constexpr (!REFINE) {
 #pragma unroll
-        for (int c = 0; c < RC; ++c)
+        for (int c = 0; c < RC; ++c) {
             if (c < n_cols && is_pt)
                 P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = gfail ? __builtin_nan("") : (double)x[c];
+            if (c < n_cols && is_poly && P.out_lambda)
+                P.out_lambda[((int64_t)c * P.n_eval + e) * M + li] = gfail ? __builtin_nan("") : (double)x[c];
+        }
     } else {
         // FP64 residual b - A x, FP32 correction on the same factors, per column
         bool converged = !gfail;
@@ -787,6 +738,8 @@ k_build(const BuildDev P)
                 }
                 if (!(rel <= P.accept_tol)) converged = false;
                 if (is_pt) P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = gfail ? __builtin_nan("") : x64;
+                if (is_poly && P.out_lambda)
+                    P.out_lambda[((int64_t)c * P.n_eval + e) * M + li] = gfail ? __builtin_nan("") : x64;
             }
         }
         if (alive) {

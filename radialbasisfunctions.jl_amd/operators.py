@@ -13,6 +13,8 @@ Mirrors (names, argument meaning and error behaviour):
 An operator is lowered to a table of terms (kind, dims, coefficient, RHS column);
 the weights are built on the GPU and kept there as CSR (one row per stencil).
 """
+import ctypes as C
+
 import numpy as np
 
 from . import _abi as abi
@@ -20,7 +22,7 @@ from . import _lib
 from .basis import AbstractRadialBasis, PHS, n_monomials, is_dirichlet
 from .errors import ArgumentError, DimensionMismatch, SingularException
 
-PRECISIONS = {"fp64": abi.PREC_FP64, "fp32_refined": abi.PREC_FP32_REFINED}
+PRECISIONS = {"fp64": abi.PREC_FP64, "fp32_refined": abi.PREC_FP32_REFINED, "learned": abi.PREC_LEARNED}
 
 
 # ============================================================== operator types
@@ -342,7 +344,7 @@ class RadialBasisOperator:
     scipy.sparse matrices (a tuple for rank >= 1)."""
 
     def __init__(self, L, data, eval_points=None, basis=None, k=None, adjl=None, hermite=None, device=None,
-                 precision="fp64", refine_steps=4, accept_tol=1e-12, _weights=None):
+                 precision="fp64", refine_steps=4, accept_tol=1e-12, model=None, _weights=None):
         if not isinstance(L, AbstractOperator):
             raise ArgumentError(f"{L!r} is not an operator")
         self.L = L
@@ -357,6 +359,9 @@ class RadialBasisOperator:
             raise DimensionMismatch("data and eval_points have different dimensions")
         self.precision = precision
         self.refine_steps, self.accept_tol = refine_steps, accept_tol
+        self.model = model
+        if precision == "learned" and model is None:
+            raise ArgumentError('precision="learned" needs a trained model (learned.LearnedModel)')
         self.k = autoselect_k(self.data, self.basis) if k is None else int(k)
         self.hermite = hermite
         self._d_data = _to_device(self.data, self.dev)
@@ -449,6 +454,10 @@ class RadialBasisOperator:
         a.refine_steps = int(self.refine_steps)
         a.accept_tol = float(self.accept_tol)
         a.stream = _lib.stream_ptr(self.dev)
+        if self.precision == "learned":
+            mstruct = self.model.struct(self.model._rho)
+            keep.append(mstruct)
+            a.model = C.addressof(mstruct)
         rc, counts = _lib.build_weights(a, self.dev)
         self.status_counts = counts
         self.status = status

@@ -23,15 +23,17 @@ def test_library_exports_declared_symbols(RBF):
     assert set(syms) <= set(RBF._lib.exported_symbols())
 
 
-def test_abi_version_without_gpu():
+def test_abi_version_without_gpu(RBF):
     lib = C.CDLL(str(ROOT / "radialbasisfunctions.jl_amd" / "librbfx.so"))
     lib.rbfx_abi_version.restype = C.c_int32
-    assert lib.rbfx_abi_version() == 1
+    assert lib.rbfx_abi_version() == RBF._abi.ABI_VERSION == 2
 
 
 def test_ctypes_mirror_sizes(RBF, O):
     assert C.sizeof(RBF._abi.BuildArgs) == O.lib().orc_sizeof_build_args()
     assert C.sizeof(RBF._abi.SpmvArgs) == O.lib().orc_sizeof_spmv_args()
+    assert C.sizeof(RBF._abi.MlpModel) == O.lib().orc_sizeof_mlp_model()
+    assert C.sizeof(RBF._abi.MlpRunArgs) == O.lib().orc_sizeof_mlp_run_args()
 
 
 def test_operator_lowering(RBF):
