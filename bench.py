@@ -3,39 +3,40 @@
 Workload: the learned-surrogate evaluation configuration: a fresh 4,000,000-point 2-D
 cloud whose density varies x10 across [0,1]^2, Laplacian with PHS(3), polynomial
 degree 2, k = 20 (n = 26 per stencil). A step is one weight build of all 4M stencils
-from the device-resident adjacency (the work of update_weights!); the k-NN and the
-apply are timed alongside and reported as extra fields.
+from the device-resident adjacency (the work of update_weights!), by each method:
+plain FP64, FP32 factorisation + FP64 refinement, and the learned surrogate with FP64
+residual acceptance and FP32-refined fallback. The k-NN and the apply (CSR SpMV) are
+timed alongside and reported as extra fields.
 
-Prints one JSON line (see DESIGN.md for every field).
+Prints one JSON line.
 """
 import argparse
 import json
 import os
+import pathlib
 import sys
 import time
 
 import numpy as np
 
-ROOT = os.path.dirname(os.path.abspath(__file__))
-sys.path.insert(0, ROOT)
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
 
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
-
-
-def density_cloud(n, seed):
-    """x from the density 1 + 9x (x10 variation), y uniform."""
-    rng = np.random.default_rng(seed)
-    u = rng.random(n)
-    # inverse CDF of p(x) = (1 + 9x) / 5.5
-    x = (-1.0 + np.sqrt(1.0 + 9.0 * 11.0 * u)) / 9.0
-    y = rng.random(n)
-    return np.stack([x, y], 1)
+MODEL_PATH = ROOT / "radialbasisfunctions.jl_amd" / "models" / "surrogate_lap2d_phs3_k20.npz"
 
 
 def flops_per_stencil(n, r):
+    """LDL^T + r triangular solves of one n x n system."""
     return n ** 3 / 3.0 + 2.0 * n * n * r
+
+
+def mlp_flops(k=20, m=6, hid=256):
+    """2 * (multiply-adds) of the 2k -> 256 -> 256 -> 256 -> k+m network."""
+    return 2 * (2 * k * hid + 2 * hid * hid + hid * (k + m))
 
 
 def main():
@@ -46,10 +47,12 @@ def main():
     ap.add_argument("--n", type=int, default=4_000_000)
     ap.add_argument("--cpu-sample", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--model", default=str(MODEL_PATH))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced "
-                                   "(see DESIGN.md for how the path would shard)", "n_gpus": args.gpus}))
+                                   "(the path shards by contiguous ranges of evaluation points)",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
@@ -57,13 +60,29 @@ def main():
     import rbfx_loader
 
     R = rbfx_loader.load()
+    L = R.learned
     dev = R._lib.device_for("cuda:0")
     k, deg = 20, 2
     m = 6
     n_sys = k + m
-    pts = density_cloud(args.n, seed=20240404)
+    pts = L.density_cloud(args.n, seed=20240404)
     d_pts = torch.as_tensor(pts, device=dev)
     stream = torch.cuda.current_stream(dev)
+
+    # surrogate: the committed model, or train + calibrate one on this GPU
+    model_info = {}
+    if pathlib.Path(args.model).exists():
+        model = L.LearnedModel.load(args.model, device=dev)
+        model_info["source"] = "loaded " + pathlib.Path(args.model).name
+    else:
+        t0 = time.time()
+        model, hist, sweep = L.train_default(device=dev, log=lambda s: print(s, file=sys.stderr, flush=True))
+        model_info.update(source="trained in this run", train_s=time.time() - t0, final_loss=hist[-1][1],
+                          calibration=sweep)
+        out = ROOT / "gpurun_out"
+        out.mkdir(exist_ok=True)
+        model.save(out / pathlib.Path(args.model).name)
+    model_info["accept_tol"] = model.accept_tol
 
     # k-NN (reported separately)
     torch.cuda.synchronize()
@@ -72,35 +91,40 @@ def main():
     torch.cuda.synchronize()
     knn_s = time.perf_counter() - t0
 
-    f = np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1])
+    f, exact = L.test_field(pts)
     d_f = torch.as_tensor(f, device=dev)
-    exact = -25.0 * f
 
     results = {}
     ops = {}
-    for method in ("fp64", "fp32_refined"):
+    stage_ms = {}
+    for method in ("fp64", "fp32_refined", "learned"):
         op = R.RadialBasisOperator(R.Laplacian(), d_pts, basis=R.PHS(3, poly_deg=deg), k=k, adjl=adj,
-                                   precision=method)
+                                   precision=method, model=model if method == "learned" else None)
         for _ in range(args.warmup):
             op.update_weights()
         torch.cuda.synchronize()
         ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        stages = np.zeros(4)
         t0 = time.perf_counter()
         ev0.record(stream)
         for _ in range(args.steps):
             op.update_weights()
+            stages += np.array(R._lib.last_build_timing())
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         gpu_ms = ev0.elapsed_time(ev1) / args.steps
+        stage_ms[method] = (stages / args.steps).tolist()
         y = op(d_f).cpu().numpy()
         nrmse = float(np.sqrt(np.mean((y - exact) ** 2)) / np.sqrt(np.mean(exact ** 2)))
+        sc = op.status_counts
         results[method] = dict(ms_per_build=gpu_ms, wall_ms_per_build=1e3 * wall / args.steps,
                                stencils_per_s=args.n / (gpu_ms * 1e-3), nrmse=nrmse,
-                               status=op.status_counts[:5])
+                               stage_ms=stage_ms[method],
+                               status={"ok": sc[0], "refined": sc[2], "fallback64": sc[3], "learned": sc[4]})
         ops[method] = op
 
-    # apply: y = W f (Laplacian, 1 field), HBM traffic per SURVEY formula
+    # apply: y = W f (Laplacian, 1 field)
     op = ops["fp64"]
     X = d_f[:, None].contiguous()
     for _ in range(3):
@@ -115,19 +139,28 @@ def main():
     torch.cuda.synchronize()
     apply_ms = ev0.elapsed_time(ev1) / reps
     nnz = args.n * k
-    apply_bytes = 4 * (args.n + 1) * 0 + nnz * (4 + 8) + 8 * (args.n + args.n)
+    apply_bytes = nnz * (4 + 8) + 8 * (args.n + args.n)  # col idx + values + x + y
     apply_gbs = apply_bytes / (apply_ms * 1e-3) / 1e9
 
     plain = results["fp64"]
-    best_name = max(results, key=lambda kk: results[kk]["stencils_per_s"])
-    best = results[best_name]
-    fl = flops_per_stencil(n_sys, 1)
-    achieved_tf = best["stencils_per_s"] * fl / 1e12
-    peak = FP64_PEAK_TFLOPS if best_name == "fp64" else FP32_PEAK_TFLOPS
+    lr = results["learned"]
+    # dominant kernel of the learned build: the stage with the most device time
+    st = stage_ms["learned"]
+    n_acc = lr["status"]["learned"]
+    n_rej = args.n - n_acc
+    if st[0] >= st[1] + st[2]:
+        dom = {"kernel": "k_learned (surrogate + FP64 acceptance)", "bound": "mfma", "unit": "TFLOP/s",
+               "peak": BF16_PEAK_TFLOPS, "per_unit": mlp_flops(), "units": args.n, "ms": st[0]}
+    else:
+        dom = {"kernel": "k_build<float> (FP32 fallback solve)", "bound": "valu-fp32 (unpacked: FP64 rate)", "unit": "TFLOP/s",
+               "peak": FP64_PEAK_TFLOPS, "per_unit": flops_per_stencil(n_sys, 1), "units": n_rej,
+               "ms": st[1]}
+    achieved = dom["per_unit"] * dom["units"] / (dom["ms"] * 1e-3) / 1e12
+    fp64_achieved = flops_per_stencil(n_sys, 1) * args.n / (stage_ms["fp64"][0] * 1e-3) / 1e12
 
     cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, str(ROOT / "oracle"))
         import oracle as O
 
         ns = min(args.cpu_sample, args.n)
@@ -139,33 +172,40 @@ def main():
         cpu_s = time.perf_counter() - t0
         cpu = {"value": ns / cpu_s, "unit": "stencils/s", "cores": threads, "kind": "port",
                "sample": f"{ns} stencils (the first {ns} eval points of the same 4M cloud), "
-                         f"C restatement of the reference arithmetic, OpenMP"}
+                         f"C restatement of the reference arithmetic (rook LDL^T per stencil), OpenMP"}
 
+    ratio = lr["nrmse"] / plain["nrmse"]
     out = {
         "metric": "weight-build stencils/s, learned+FP32-refined vs plain FP64, at equal operator err",
-        "value": best["stencils_per_s"],
+        "value": lr["stencils_per_s"],
         "unit": "stencils/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": best["ms_per_build"],
+        "ms_per_step": lr["ms_per_build"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if best_name == "fp64" else "f32+f64-refine",
+        "dtype": "bf16-mfma+f64-accept / f32+f64-refine fallback",
         "data": "synthetic (seeded 2-D cloud, density x10 across the domain)",
-        "config": {"workload": "laplacian-2d-4M-phs3-deg2-k20", "n_points": args.n, "k": k, "poly_deg": deg,
-                   "basis": "PHS3", "method": best_name},
+        "config": {"workload": "laplacian-2d-4M-phs3-deg2-k20-learned", "n_points": args.n, "k": k,
+                   "poly_deg": deg, "basis": "PHS3", "method": "learned"},
+        "speedup_vs_plain_fp64": lr["stencils_per_s"] / plain["stencils_per_s"],
+        "error_ratio_vs_plain": ratio,
+        "equal_error_met": bool(ratio <= 1.05),
+        "accepted_fraction": n_acc / args.n,
         "methods": results,
-        "speedup_vs_plain_fp64": best["stencils_per_s"] / plain["stencils_per_s"],
-        "error_ratio_vs_plain": best["nrmse"] / plain["nrmse"],
-        "learned": None,
+        "model": model_info,
         "knn_ms": knn_s * 1e3,
         "apply": {"ms": apply_ms, "GB_s": apply_gbs, "frac_hbm_peak": apply_gbs / HBM_PEAK_GBS,
                   "bytes": apply_bytes},
-        "roofline": {"bound": "valu-fp64" if best_name == "fp64" else "valu-fp32", "achieved": achieved_tf,
-                     "peak": peak, "unit": "TFLOP/s", "frac": achieved_tf / peak, "traffic": None,
-                     "flops_per_stencil": fl},
+        "roofline": {"bound": dom["bound"], "achieved": achieved, "peak": dom["peak"], "unit": dom["unit"],
+                     "frac": achieved / dom["peak"], "traffic": None, "kernel": dom["kernel"],
+                     "per_unit_flops": dom["per_unit"], "units_per_launch": dom["units"],
+                     "kernel_ms": dom["ms"]},
+        "plain_fp64_roofline": {"bound": "valu-fp64", "achieved": fp64_achieved, "peak": FP64_PEAK_TFLOPS,
+                                "unit": "TFLOP/s", "frac": fp64_achieved / FP64_PEAK_TFLOPS,
+                                "per_unit_flops": flops_per_stencil(n_sys, 1), "kernel_ms": stage_ms["fp64"][0]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

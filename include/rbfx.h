@@ -245,6 +245,13 @@ int rbfx_spmv(const rbfx_spmv_args* args);
  * thread (counts[0..15]); lets the host raise the reference's exception types. */
 int rbfx_last_build_counts(int64_t counts[16]);
 
+/* Device time (ms, HIP events on the build's stream) of the stages of the last
+ * rbfx_build_weights on this thread: [0] the main kernel (plain solve, FP32 solve
+ * or surrogate), [1] the FP32 solve of the surrogate's rejected stencils, [2] the
+ * FP64 re-solve of non-converged stencils, [3] the whole build. No reference
+ * counterpart (instrumentation).                                                  */
+int rbfx_last_build_timing(double ms[4]);
+
 /* ---- surrogate training and inputs (no reference counterpart) ------------ */
 
 /* Sizes of the flat FP32 parameter array, its BF16 weight copy, and the run

@@ -52,6 +52,8 @@ def lib():
                 L.rbfx_spmv.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_last_build_counts.restype = C.c_int
                 L.rbfx_last_build_counts.argtypes = [C.POINTER(C.c_int64)]
+                L.rbfx_last_build_timing.restype = C.c_int
+                L.rbfx_last_build_timing.argtypes = [C.POINTER(C.c_double)]
                 L.rbfx_mlp_param_count.restype = C.c_int64
                 L.rbfx_mlp_bf16_count.restype = C.c_int64
                 L.rbfx_mlp_workspace_bytes.restype = C.c_int64
@@ -72,7 +74,7 @@ def lib():
 
 def exported_symbols():
     return ["rbfx_init", "rbfx_last_error", "rbfx_abi_version", "rbfx_knn", "rbfx_build_weights",
-            "rbfx_spmv", "rbfx_last_build_counts", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
+            "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
             "rbfx_stencil_features"]
 
@@ -199,3 +201,10 @@ def stencil_features(data, eval_points, adj, k, want_xi=True, want_x=True, want_
         _check(lib().rbfx_stencil_features(ptr(data), ptr(eval_points), n, data.shape[1], int(k), ptr(adj), ptr(xi),
                                            ptr(x), ptr(h), stream_ptr(dev)))
     return xi, x, h
+
+
+def last_build_timing():
+    """Device ms of the last build's stages: main kernel, FP32 list solve, FP64 fallback, total."""
+    ms = (C.c_double * 4)()
+    lib().rbfx_last_build_timing(ms)
+    return list(ms)

@@ -25,6 +25,7 @@ namespace {
 
 thread_local std::string g_err;
 thread_local int64_t g_counts[16];
+thread_local double g_stage_ms[4];  // last build: main kernel, FP32 list solve, FP64 fallback, total
 
 int set_err(int code, const char* fmt, ...)
 {
@@ -55,6 +56,7 @@ struct DeviceState {
     int32_t* reject_list = nullptr;
     size_t cap_reject = 0;
     int32_t* reject_count = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // stage boundaries of a build
 };
 
 std::mutex g_mu;
@@ -142,6 +144,12 @@ int rbfx_last_build_counts(int64_t counts[16])
     return RBFX_OK;
 }
 
+int rbfx_last_build_timing(double ms[4])
+{
+    for (int i = 0; i < 4; ++i) ms[i] = g_stage_ms[i];
+    return RBFX_OK;
+}
+
 int rbfx_knn(const double* data, int64_t n_data, const double* query, int64_t n_query, int32_t dim, int32_t k,
              const int64_t* q_ptr, int32_t* out_idx, void* stream)
 {
@@ -162,6 +170,7 @@ int rbfx_build_weights(const rbfx_build_args* a)
 {
     if (!a) return set_err(RBFX_EINVAL, "rbfx_build_weights: null args");
     for (int i = 0; i < 16; ++i) g_counts[i] = 0;
+    for (int i = 0; i < 4; ++i) g_stage_ms[i] = 0.0;
     if (a->dim < 1 || a->dim > 3) return set_err(RBFX_EINVAL, "dim must be 1, 2 or 3 (got %d)", a->dim);
     if (!a->data || !a->eval || !a->adj_idx || !a->out_val)
         return set_err(RBFX_EINVAL, "rbfx_build_weights: data, eval, adj_idx and out_val are required");
@@ -285,8 +294,14 @@ int rbfx_build_weights(const rbfx_build_args* a)
     P.ok_status = RBFX_ST_OK;
 
     const int nmax = a->k + M;
+    for (int i = 0; i < 4; ++i)
+        if (!ds.ev[i]) HIPCHK(hipEventCreate(&ds.ev[i]));
+    // stage boundaries: ev0 | main kernel | ev1 | FP32 list solve | ev2 | FP64 fallback | ev3
+    HIPCHK(hipEventRecord(ds.ev[0], st));
     if (a->precision == RBFX_PREC_FP64) {
         HIPCHK(rbfx::launch_build(P, a->dim, nmax, false, a->n_eval, st));
+        HIPCHK(hipEventRecord(ds.ev[1], st));
+        HIPCHK(hipEventRecord(ds.ev[2], st));
     } else {
         HIPCHK(grow((void**)&ds.fail_list, ds.cap_fail, sizeof(int32_t) * (size_t)a->n_eval));
         if (!ds.fail_count) HIPCHK(hipMalloc((void**)&ds.fail_count, sizeof(int32_t)));
@@ -325,6 +340,7 @@ int rbfx_build_weights(const rbfx_build_args* a)
             Q.reject_count = ds.reject_count;
             Q.out_rho = mdl->out_rho;
             HIPCHK(rbfx::launch_learned_build(Q, st));
+            HIPCHK(hipEventRecord(ds.ev[1], st));
             int32_t nrej = 0;
             HIPCHK(read_count(ds.reject_count, &nrej, st));
             P.list = ds.reject_list;
@@ -332,6 +348,9 @@ int rbfx_build_weights(const rbfx_build_args* a)
             n_solve = nrej;
         }
         if (n_solve > 0) HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, n_solve, st));
+        // FP32_REFINED: the FP32 solve is the main kernel (stage 1 stays empty)
+        if (a->precision != RBFX_PREC_LEARNED) HIPCHK(hipEventRecord(ds.ev[1], st));
+        HIPCHK(hipEventRecord(ds.ev[2], st));
         // FP64 re-solve of the stencils whose refinement did not converge
         int32_t nfail = 0;
         HIPCHK(read_count(ds.fail_count, &nfail, st));
@@ -343,12 +362,23 @@ int rbfx_build_weights(const rbfx_build_args* a)
         F.ok_status = RBFX_ST_FALLBACK64;
         if (nfail > 0) HIPCHK(rbfx::launch_build(F, a->dim, nmax, false, nfail, st));
     }
+    HIPCHK(hipEventRecord(ds.ev[3], st));
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ds.hist, 16);
     const int64_t hb = std::min<int64_t>((a->n_eval + 255) / 256, 4096);
     hipLaunchKernelGGL(k_hist16, dim3((unsigned)hb), dim3(256), 0, st, status, a->n_eval, ds.hist);
     unsigned long long h[16];
     HIPCHK(hipMemcpyAsync(h, ds.hist, sizeof(h), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < 3; ++i) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ds.ev[i], ds.ev[i + 1]));
+        g_stage_ms[i] = ms;
+    }
+    {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ds.ev[0], ds.ev[3]));
+        g_stage_ms[3] = ms;
+    }
     for (int i = 0; i < 16; ++i) g_counts[i] = (int64_t)h[i];
     if (h[RBFX_ST_TOO_LARGE])
         return set_err(RBFX_EUNSUPPORTED, "%llu stencils exceed the device maximum k+m=%d", h[RBFX_ST_TOO_LARGE], RBFX_MAX_N);

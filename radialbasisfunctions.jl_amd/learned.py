@@ -219,3 +219,55 @@ def weight_errors(model, X, Y):
     num = (P[:, :k] - w_true).abs().amax(1)
     den = w_true.abs().amax(1)
     return (num / den).cpu().numpy()
+
+
+# ---- calibration of the acceptance tolerance -------------------------------------
+def test_field(pts):
+    """f = sin(4x) cos(3y) and its Laplacian -25 f."""
+    f = np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1])
+    return f, -25.0 * f
+
+
+def calibrate(model, n=200_000, seed=97531, tols=(0.2, 0.1, 0.05, 0.03, 0.02, 0.01, 0.005, 0.002, 0.001),
+              target_ratio=1.03, device=None, log=print):
+    """Pick the largest acceptance tolerance whose applied Laplacian error on a held-out
+    density cloud stays within `target_ratio` of the plain FP64 build's (f = sin4x cos3y).
+    Sets model.accept_tol and returns the sweep [(tol, ratio, accepted fraction)]."""
+    from . import operators as ops
+    from .basis import PHS
+
+    dev = _lib.device_for(device)
+    pts = density_cloud(n, seed)
+    f, ex = test_field(pts)
+    plain = ops.laplacian(pts, basis=PHS(3, poly_deg=2), k=model.k, device=dev, precision="fp64")
+
+    def nrmse(y):
+        return float(np.sqrt(np.mean((y - ex) ** 2)) / np.sqrt(np.mean(ex ** 2)))
+
+    e0 = nrmse(plain(f))
+    sweep = []
+    chosen = None
+    for tol in sorted(tols, reverse=True):
+        model.accept_tol = tol
+        op = ops.laplacian(pts, basis=PHS(3, poly_deg=2), k=model.k, device=dev, adjl=plain.adj_idx,
+                           precision="learned", model=model)
+        ratio = nrmse(op(f)) / e0
+        acc = op.status_counts[abi.ST_LEARNED] / n
+        sweep.append((tol, ratio, acc))
+        if log:
+            log(f"  tol {tol:.3g}: error ratio {ratio:.4f}, accepted {acc:.3f}")
+        if chosen is None and ratio <= target_ratio:
+            chosen = tol
+    model.accept_tol = chosen if chosen is not None else min(tols)
+    return sweep
+
+
+def train_default(device=None, n_train=2_000_000, steps=20000, batch=4096, seed=2024, log=print):
+    """The 2-D Laplacian surrogate (PHS3, degree 2, k=20): 2M training stencils from
+    seeded density clouds, Adam with a cosine schedule, then tolerance calibration."""
+    dev = _lib.device_for(device)
+    X, Y = training_set(n_train, seed=seed, device=dev)
+    model, hist = train(X, Y, steps=steps, batch=batch, seed=seed, device=dev, log=log)
+    del X, Y
+    sweep = calibrate(model, device=dev, log=log)
+    return model, hist, sweep
