@@ -212,6 +212,10 @@ typedef struct rbfx_spmv_args {
     double alpha;
     double beta;
     void* stream;
+    const int32_t* row_order;    /* ABI 2, optional n_rows: the order in which rows are
+                                    processed (e.g. from rbfx_spatial_order) so the X
+                                    gathers of consecutive rows share cache lines; the
+                                    result does not depend on it                        */
 } rbfx_spmv_args;
 
 /* ---- entry points -------------------------------------------------------- */
@@ -240,6 +244,12 @@ int rbfx_build_weights(const rbfx_build_args* args);
 /* op.weights * X (operators.jl:217-218) and mul!(y, op, x, alpha, beta)
  * (operators.jl:536-542) for one or several value arrays sharing a pattern.      */
 int rbfx_spmv(const rbfx_spmv_args* args);
+
+/* Morton (Z-curve) order of n points (FP64 AoS, dim doubles each): out_order[r]
+ * is the index of the r-th point along the curve. The device-side counterpart of
+ * reorder_points! (src/utils.jl:45-55), used to order the apply's rows without
+ * moving user data.                                                               */
+int rbfx_spatial_order(const double* points, int64_t n, int32_t dim, int32_t* out_order, void* stream);
 
 /* Number of stencils per status code from the last rbfx_build_weights on this
  * thread (counts[0..15]); lets the host raise the reference's exception types. */

@@ -185,6 +185,7 @@ class SpmvArgs(C.Structure):
         ("alpha", C.c_double),
         ("beta", C.c_double),
         ("stream", C.c_void_p),
+        ("row_order", C.c_void_p),
     ]
 
 

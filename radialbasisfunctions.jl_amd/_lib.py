@@ -52,6 +52,8 @@ def lib():
                 L.rbfx_spmv.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_last_build_counts.restype = C.c_int
                 L.rbfx_last_build_counts.argtypes = [C.POINTER(C.c_int64)]
+                L.rbfx_spatial_order.restype = C.c_int
+                L.rbfx_spatial_order.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
                 L.rbfx_last_build_timing.restype = C.c_int
                 L.rbfx_last_build_timing.argtypes = [C.POINTER(C.c_double)]
                 L.rbfx_mlp_param_count.restype = C.c_int64
@@ -76,7 +78,7 @@ def exported_symbols():
     return ["rbfx_init", "rbfx_last_error", "rbfx_abi_version", "rbfx_knn", "rbfx_build_weights",
             "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
-            "rbfx_stencil_features"]
+            "rbfx_stencil_features", "rbfx_spatial_order"]
 
 
 def _check(rc):
@@ -208,3 +210,13 @@ def last_build_timing():
     ms = (C.c_double * 4)()
     lib().rbfx_last_build_timing(ms)
     return list(ms)
+
+
+def spatial_order(points):
+    """Morton order of device points (n, dim) f64 -> int32 (n,) device permutation."""
+    t = torch()
+    dev = points.device
+    out = t.empty(points.shape[0], dtype=t.int32, device=dev)
+    with t.cuda.device(dev):
+        _check(lib().rbfx_spatial_order(ptr(points), points.shape[0], points.shape[1], ptr(out), stream_ptr(dev)))
+    return out

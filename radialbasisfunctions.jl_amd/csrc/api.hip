@@ -18,6 +18,7 @@
 #include "knn.h"
 #include "learned.h"
 #include "mlp.h"
+#include "order.h"
 #include "rbfx.h"
 #include "spmv.h"
 
@@ -560,6 +561,15 @@ int rbfx_stencil_features(const double* data, const double* eval, int64_t n_eval
     return RBFX_OK;
 }
 
+int rbfx_spatial_order(const double* points, int64_t n, int32_t dim, int32_t* out_order, void* stream)
+{
+    if (!points || !out_order) return set_err(RBFX_EINVAL, "rbfx_spatial_order: null pointer argument");
+    if (dim < 1 || dim > 3) return set_err(RBFX_EINVAL, "rbfx_spatial_order: dim must be 1, 2 or 3");
+    if (n > INT32_MAX) return set_err(RBFX_EUNSUPPORTED, "rbfx_spatial_order: n above 2^31-1");
+    HIPCHK(rbfx::spatial_order(points, n, dim, out_order, (hipStream_t)stream));
+    return RBFX_OK;
+}
+
 int rbfx_spmv(const rbfx_spmv_args* a)
 {
     if (!a) return set_err(RBFX_EINVAL, "rbfx_spmv: null args");
@@ -584,6 +594,7 @@ int rbfx_spmv(const rbfx_spmv_args* a)
     P.y_ld_val = a->y_ld_val;
     P.alpha = a->alpha;
     P.beta = a->beta;
+    P.row_order = a->row_order;
     for (int v0 = 0; v0 < a->n_valarrays; v0 += 4) {
         const int nv = std::min(4, a->n_valarrays - v0);
         P.n_valarrays = nv;

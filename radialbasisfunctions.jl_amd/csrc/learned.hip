@@ -62,7 +62,6 @@ __device__ __forceinline__ void mlp_tiles(const __bf16 (*act)[kActLd], const __b
 #pragma unroll
         for (int j = 0; j < NT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int fr = lane & 15, fk = (lane >> 4) * 8;
-#pragma unroll 2
     for (int k0 = 0; k0 < KL; k0 += 32) {
         bf16x8 a[4], b[NT];
 #pragma unroll
@@ -98,7 +97,8 @@ __device__ __forceinline__ void hidden_layer(__bf16 (*act)[kActLd], const __bf16
 }
 
 template <int K, int MM>
-__global__ __launch_bounds__(256) void k_learned(const LearnedDev P)
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(2))) void k_learned(
+    const LearnedDev P)
 {
     constexpr int R = (K + 3) / 4;  // rows per quad lane
     __shared__ __attribute__((aligned(16))) __bf16 s_act[kBM][kActLd];
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(256) void k_learned(const LearnedDev P)
     BasisDev B;
     B.kind = P.basis_kind;
     B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
-    double wt[R], Pr[R][MM];
+    double wt[R];
     double ptw[MM], G[MM][MM];
 #pragma unroll
     for (int a = 0; a < MM; ++a) {
@@ -217,17 +217,16 @@ __global__ __launch_bounds__(256) void k_learned(const LearnedDev P)
     for (int r = 0; r < R; ++r) {
         const int i = q + 4 * r;
         wt[r] = 0.0;
-#pragma unroll
-        for (int a = 0; a < MM; ++a) Pr[r][a] = 0.0;
         if (i < K) {
             wt[r] = (double)s_out[sl][i] * P.out_std[i] + P.out_mean[i];
+            double pr[MM];
 #pragma unroll
-            for (int a = 0; a < MM; ++a) Pr[r][a] = mono2(s_mono[a], dx[r], dy[r]);
+            for (int a = 0; a < MM; ++a) pr[a] = mono2(s_mono[a], dx[r], dy[r]);
 #pragma unroll
             for (int a = 0; a < MM; ++a) {
-                ptw[a] += Pr[r][a] * wt[r];
+                ptw[a] += pr[a] * wt[r];
 #pragma unroll
-                for (int b = 0; b <= a; ++b) G[a][b] += Pr[r][a] * Pr[r][b];
+                for (int b = 0; b <= a; ++b) G[a][b] += pr[a] * pr[b];
             }
         }
     }
@@ -289,7 +288,7 @@ __global__ __launch_bounds__(256) void k_learned(const LearnedDev P)
         const int i = q + 4 * r;
         double corr = 0.0;
 #pragma unroll
-        for (int a = 0; a < MM; ++a) corr += Pr[r][a] * rhs[a];
+        for (int a = 0; a < MM; ++a) corr += mono2(s_mono[a], dx[r], dy[r]) * rhs[a];
         wt[r] += corr;
         if (i < K) s_w[sl][i] = wt[r];
     }
@@ -320,7 +319,7 @@ __global__ __launch_bounds__(256) void k_learned(const LearnedDev P)
             }
             sres[r] = lphi - acc;
 #pragma unroll
-            for (int a = 0; a < MM; ++a) pts[a] += Pr[r][a] * sres[r];
+            for (int a = 0; a < MM; ++a) pts[a] += mono2(s_mono[a], dx[r], dy[r]) * sres[r];
         }
     }
 #pragma unroll
@@ -332,7 +331,7 @@ __global__ __launch_bounds__(256) void k_learned(const LearnedDev P)
         if (q + 4 * r < K) {
             double v = sres[r];
 #pragma unroll
-            for (int a = 0; a < MM; ++a) v -= Pr[r][a] * pts[a];
+            for (int a = 0; a < MM; ++a) v -= mono2(s_mono[a], dx[r], dy[r]) * pts[a];
             rmax = fmax(rmax, fabs(v));
         }
     }

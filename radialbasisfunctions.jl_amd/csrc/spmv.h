@@ -19,6 +19,7 @@ struct SpmvDev {
     double* y;
     int64_t y_ld_point, y_ld_field, y_ld_val;
     double alpha, beta;
+    const int32_t* row_order;  // optional: processing order of the rows (spatial order)
 };
 
 hipError_t launch_spmv(const SpmvDev& P, hipStream_t s);

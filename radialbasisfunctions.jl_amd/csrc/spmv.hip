@@ -20,14 +20,30 @@
 
 namespace rbfx {
 
+// Workgroups are dealt to the 8 XCDs round-robin; remap so each XCD walks one
+// contiguous range of (spatially ordered) rows and its own L2 serves their gathers.
+// The grid is padded to a multiple of 8; blocks mapped past the end return -1.
+__device__ __forceinline__ int64_t xcd_block(int64_t nb_real)
+{
+    const int64_t nb = gridDim.x;  // multiple of 8
+    const int64_t b = blockIdx.x;
+    const int64_t lb = (b & 7) * (nb >> 3) + (b >> 3);
+    return lb < nb_real ? lb : -1;
+}
+
+__host__ inline int64_t pad8(int64_t nb) { return (nb + 7) / 8 * 8; }
+
 template <int NV>
-__global__ __launch_bounds__(256) void k_spmv_fields(const SpmvDev P, int G /* pow2 >= F */)
+__global__ __launch_bounds__(256) void k_spmv_fields(const SpmvDev P, int G /* pow2 >= F */, int64_t nb_real)
 {
     const int lane = threadIdx.x;
     const int f = lane % G;
     const int rows_per_block = blockDim.x / G;
-    const int64_t row = (int64_t)blockIdx.x * rows_per_block + lane / G;
-    if (row >= P.n_rows || f >= P.n_fields) return;
+    const int64_t lb = xcd_block(nb_real);
+    if (lb < 0) return;
+    const int64_t r = lb * rows_per_block + lane / G;
+    if (r >= P.n_rows || f >= P.n_fields) return;
+    const int64_t row = P.row_order ? (int64_t)P.row_order[r] : r;
     const int64_t b = P.row_ptr ? P.row_ptr[row] : row * (int64_t)P.k;
     const int64_t e = P.row_ptr ? P.row_ptr[row + 1] : b + P.k;
     double acc[NV];
@@ -49,11 +65,14 @@ __global__ __launch_bounds__(256) void k_spmv_fields(const SpmvDev P, int G /* p
 }
 
 template <int NV, int G, int FMAX>
-__global__ __launch_bounds__(256) void k_spmv_rows(const SpmvDev P)
+__global__ __launch_bounds__(256) void k_spmv_rows(const SpmvDev P, int64_t nb_real)
 {
     const int sub = threadIdx.x % G;
-    const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-    const bool live = row < P.n_rows;
+    const int64_t lb = xcd_block(nb_real);
+    if (lb < 0) return;  // whole block: the butterfly below stays inside live blocks
+    const int64_t r = (lb * blockDim.x + threadIdx.x) / G;
+    const bool live = r < P.n_rows;
+    const int64_t row = live ? (P.row_order ? (int64_t)P.row_order[r] : r) : 0;
     double acc[NV][FMAX];
 #pragma unroll
     for (int v = 0; v < NV; ++v)
@@ -105,13 +124,16 @@ static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
         if (G > 64) return hipErrorInvalidValue;
         const int rows_per_block = TB / G;
         const int64_t grid = (P.n_rows + rows_per_block - 1) / rows_per_block;
-        hipLaunchKernelGGL(k_spmv_fields<NV>, dim3((unsigned)grid), dim3(TB), 0, s, P, G);
+        hipLaunchKernelGGL(k_spmv_fields<NV>, dim3((unsigned)pad8(grid)), dim3(TB), 0, s, P, G, grid);
         return hipGetLastError();
     }
     const int64_t threads_g4 = P.n_rows * 4, threads_g8 = P.n_rows * 8, threads_g16 = P.n_rows * 16;
-#define RBFX_SPMV_LAUNCH(GG, FF, TH)                                                                  \
-    hipLaunchKernelGGL((k_spmv_rows<NV, GG, FF>), dim3((unsigned)(((TH) + TB - 1) / TB)), dim3(TB), 0, s, P); \
-    return hipGetLastError();
+#define RBFX_SPMV_LAUNCH(GG, FF, TH)                                                                   \
+    {                                                                                                  \
+        const int64_t nbr = ((TH) + TB - 1) / TB;                                                      \
+        hipLaunchKernelGGL((k_spmv_rows<NV, GG, FF>), dim3((unsigned)pad8(nbr)), dim3(TB), 0, s, P, nbr); \
+        return hipGetLastError();                                                                      \
+    }
     if (P.n_fields <= 1) {
         if (avg_row < 12) { RBFX_SPMV_LAUNCH(4, 1, threads_g4) }
         if (avg_row < 40) { RBFX_SPMV_LAUNCH(8, 1, threads_g8) }

@@ -343,6 +343,8 @@ class RadialBasisOperator:
     (operators.jl:50-159). Weights live on the GPU as CSR; `weights()` returns
     scipy.sparse matrices (a tuple for rank >= 1)."""
 
+    SPATIAL_ORDER_MIN = 65536  # below this the X vector sits in L2 anyway
+
     def __init__(self, L, data, eval_points=None, basis=None, k=None, adjl=None, hermite=None, device=None,
                  precision="fp64", refine_steps=4, accept_tol=1e-12, model=None, _weights=None):
         if not isinstance(L, AbstractOperator):
@@ -377,6 +379,7 @@ class RadialBasisOperator:
             self.adj_idx = _to_device(idx, self.dev, t.int32)
         self.status_counts = None
         self._host_weights = None
+        self._row_order = None
         if _weights is None:
             self._build()
         else:
@@ -528,6 +531,11 @@ class RadialBasisOperator:
         s.alpha = float(alpha)
         s.beta = float(beta)
         s.stream = _lib.stream_ptr(self.dev)
+        if self._row_order is None and self.n_eval >= self.SPATIAL_ORDER_MIN:
+            # rows walked along a Z-curve of the eval points: neighbouring rows gather
+            # neighbouring X entries (the result does not depend on the order)
+            self._row_order = _lib.spatial_order(self._d_eval)
+        s.row_order = _lib.ptr(self._row_order)
         _lib.spmv(s, self.dev)
         return Y
 

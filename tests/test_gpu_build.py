@@ -306,3 +306,38 @@ def test_built_operator_algebra(RBF, gpu):
     other = RBF.partial(pts[:1000], 2, 1)
     with pytest.raises(RBF.ArgumentError):
         dxx + other
+
+
+def test_spatial_order_and_ordered_apply(RBF, gpu):
+    """The Morton order is a permutation that keeps neighbours close, and the apply
+    gives bit-identical results with and without it (it only changes which rows a
+    workgroup processes together)."""
+    import torch
+
+    rng = np.random.default_rng(5)
+    pts = rng.random((100_000, 2))
+    d = torch.as_tensor(pts, device=gpu)
+    order = RBF._lib.spatial_order(d).cpu().numpy()
+    assert np.array_equal(np.sort(order), np.arange(len(pts)))
+    # consecutive points along the curve are much closer than random pairs
+    step = np.sqrt(((pts[order[1:]] - pts[order[:-1]]) ** 2).sum(1))
+    assert np.median(step) < 0.05 * np.median(np.sqrt(((pts[1:] - pts[:-1]) ** 2).sum(1)))
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
+    assert op._row_order is None
+    f = torch.as_tensor(np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1]), device=gpu)
+    y1 = op(f).cpu().numpy()
+    assert op._row_order is not None  # n_eval above the threshold: ordered apply
+    keep = op._row_order
+    op._row_order = None
+    op.SPATIAL_ORDER_MIN = 10 ** 12
+    y0 = op(f).cpu().numpy()
+    assert np.array_equal(y0, y1)
+    op._row_order = keep
+    # multi-field, two value arrays (gradient) through the ordered path
+    g = RBF.gradient(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
+    F = torch.as_tensor(rng.random((len(pts), 3)), device=gpu)
+    G1 = g(F).cpu().numpy()
+    W = g.weights()
+    for c in range(2):
+        ref = W[c] @ F.cpu().numpy()
+        assert np.allclose(G1[..., c], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
