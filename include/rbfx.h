@@ -245,6 +245,15 @@ int rbfx_build_weights(const rbfx_build_args* args);
  * (operators.jl:536-542) for one or several value arrays sharing a pattern.      */
 int rbfx_spmv(const rbfx_spmv_args* args);
 
+/* Capturable variants (no reference counterpart): the same work, enqueued on
+ * args->stream without synchronising, reading back or allocating, so a caller can
+ * record build + apply in a HIP graph. The asynchronous build reports no status
+ * counts and skips the host-side Hermite normal-form check; it needs one earlier
+ * blocking rbfx_build_weights with the same sizes (which sizes the library's lists).
+ * Its fallback launches cover every stencil and read the device counters.        */
+int rbfx_build_weights_async(const rbfx_build_args* args);
+int rbfx_spmv_async(const rbfx_spmv_args* args);
+
 /* Morton (Z-curve) order of n points (FP64 AoS, dim doubles each): out_order[r]
  * is the index of the r-th point along the curve. The device-side counterpart of
  * reorder_points! (src/utils.jl:45-55), used to order the apply's rows without

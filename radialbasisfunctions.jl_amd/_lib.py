@@ -50,6 +50,10 @@ def lib():
                 L.rbfx_build_weights.argtypes = [C.POINTER(abi.BuildArgs)]
                 L.rbfx_spmv.restype = C.c_int
                 L.rbfx_spmv.argtypes = [C.POINTER(abi.SpmvArgs)]
+                L.rbfx_build_weights_async.restype = C.c_int
+                L.rbfx_build_weights_async.argtypes = [C.POINTER(abi.BuildArgs)]
+                L.rbfx_spmv_async.restype = C.c_int
+                L.rbfx_spmv_async.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_last_build_counts.restype = C.c_int
                 L.rbfx_last_build_counts.argtypes = [C.POINTER(C.c_int64)]
                 L.rbfx_spatial_order.restype = C.c_int
@@ -78,7 +82,7 @@ def exported_symbols():
     return ["rbfx_init", "rbfx_last_error", "rbfx_abi_version", "rbfx_knn", "rbfx_build_weights",
             "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
-            "rbfx_stencil_features", "rbfx_spatial_order"]
+            "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async"]
 
 
 def _check(rc):
@@ -142,10 +146,17 @@ def build_weights(args, dev):
     return rc, list(counts)
 
 
-def spmv(args, dev):
+def spmv(args, dev, asynchronous=False):
     t = torch()
     with t.cuda.device(dev):
-        _check(lib().rbfx_spmv(C.byref(args)))
+        f = lib().rbfx_spmv_async if asynchronous else lib().rbfx_spmv
+        _check(f(C.byref(args)))
+
+
+def build_weights_async(args, dev):
+    t = torch()
+    with t.cuda.device(dev):
+        _check(lib().rbfx_build_weights_async(C.byref(args)))
 
 
 def last_error():

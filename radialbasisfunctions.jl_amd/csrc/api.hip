@@ -119,55 +119,44 @@ hipError_t read_count(const int32_t* d, int32_t* h, hipStream_t st)
 
 }  // namespace
 
-extern "C" {
-
-int32_t rbfx_abi_version(void) { return 2; }
-
-const char* rbfx_last_error(void) { return g_err.c_str(); }
-
-int rbfx_init(int32_t device)
+static int spmv_impl(const rbfx_spmv_args* a, bool async)
 {
-    int n = 0;
-    HIPCHK(hipGetDeviceCount(&n));
-    if (device < 0 || device >= n) return set_err(RBFX_EINVAL, "device %d out of range (%d devices)", device, n);
-    HIPCHK(hipSetDevice(device));
-    hipDeviceProp_t prop;
-    HIPCHK(hipGetDeviceProperties(&prop, device));
-    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
-        return set_err(RBFX_EDEVICE, "rbfx kernels are built for gfx950; device %d is %s", device, prop.gcnArchName);
-    g_err.clear();
+    if (!a) return set_err(RBFX_EINVAL, "rbfx_spmv: null args");
+    if (!a->col_idx || !a->vals || !a->x || !a->y) return set_err(RBFX_EINVAL, "rbfx_spmv: null pointer argument");
+    if (a->n_fields < 1 || a->n_fields > 64) return set_err(RBFX_EUNSUPPORTED, "rbfx_spmv: n_fields=%d out of range 1..64", a->n_fields);
+    if (a->n_valarrays < 1) return set_err(RBFX_EINVAL, "rbfx_spmv: n_valarrays must be >= 1");
+    if (!a->row_ptr && a->k < 1) return set_err(RBFX_EINVAL, "rbfx_spmv: k must be >= 1 without row_ptr");
+    if (a->n_rows == 0) return RBFX_OK;
+    hipStream_t st = (hipStream_t)a->stream;
+    rbfx::SpmvDev P;
+    P.n_rows = a->n_rows;
+    P.row_ptr = a->row_ptr;
+    P.k = a->k;
+    P.col_idx = a->col_idx;
+    P.nnz = a->nnz > 0 ? a->nnz : a->n_rows * (int64_t)a->k;
+    P.n_fields = a->n_fields;
+    P.x = a->x;
+    P.x_ld_point = a->x_ld_point;
+    P.x_ld_field = a->x_ld_field;
+    P.y_ld_point = a->y_ld_point;
+    P.y_ld_field = a->y_ld_field;
+    P.y_ld_val = a->y_ld_val;
+    P.alpha = a->alpha;
+    P.beta = a->beta;
+    P.row_order = a->row_order;
+    for (int v0 = 0; v0 < a->n_valarrays; v0 += 4) {
+        const int nv = std::min(4, a->n_valarrays - v0);
+        P.n_valarrays = nv;
+        P.vals = a->vals + (int64_t)v0 * P.nnz;
+        P.y = a->y + (int64_t)v0 * a->y_ld_val;
+        HIPCHK(rbfx::launch_spmv(P, st));
+    }
+    if (!async) HIPCHK(hipStreamSynchronize(st));
     return RBFX_OK;
 }
 
-int rbfx_last_build_counts(int64_t counts[16])
-{
-    for (int i = 0; i < 16; ++i) counts[i] = g_counts[i];
-    return RBFX_OK;
-}
 
-int rbfx_last_build_timing(double ms[4])
-{
-    for (int i = 0; i < 4; ++i) ms[i] = g_stage_ms[i];
-    return RBFX_OK;
-}
-
-int rbfx_knn(const double* data, int64_t n_data, const double* query, int64_t n_query, int32_t dim, int32_t k,
-             const int64_t* q_ptr, int32_t* out_idx, void* stream)
-{
-    if (!data || !query || !out_idx) return set_err(RBFX_EINVAL, "rbfx_knn: null pointer argument");
-    if (dim < 1 || dim > 3) return set_err(RBFX_EINVAL, "rbfx_knn: dim must be 1, 2 or 3 (got %d)", dim);
-    if (k < 1 || k > n_data) return set_err(RBFX_EINVAL, "rbfx_knn: need 1 <= k <= n_data (k=%d, n_data=%lld)", k, (long long)n_data);
-    if (k > 64) return set_err(RBFX_EUNSUPPORTED, "rbfx_knn: k=%d exceeds the supported maximum of 64", k);
-    if (n_query == 0) return RBFX_OK;
-    hipStream_t st = (hipStream_t)stream;
-    std::lock_guard<std::mutex> lk(g_mu);
-    DeviceState& ds = dev_state();
-    HIPCHK(rbfx::run_knn(ds.knn, data, n_data, query, n_query, dim, k, q_ptr, out_idx, st));
-    HIPCHK(hipStreamSynchronize(st));
-    return RBFX_OK;
-}
-
-int rbfx_build_weights(const rbfx_build_args* a)
+static int build_impl(const rbfx_build_args* a, bool async)
 {
     if (!a) return set_err(RBFX_EINVAL, "rbfx_build_weights: null args");
     for (int i = 0; i < 16; ++i) g_counts[i] = 0;
@@ -228,7 +217,7 @@ int rbfx_build_weights(const rbfx_build_args* a)
     if (a->is_boundary) {
         if (!a->bnd_index || (a->n_bnd > 0 && (!a->bc_alpha || !a->bc_beta || !a->normals)))
             return set_err(RBFX_EINVAL, "Hermite build needs bnd_index, bc_alpha, bc_beta and normals");
-        if (a->n_bnd > 0) {
+        if (a->n_bnd > 0 && !async) {
             std::vector<double> al(a->n_bnd), be(a->n_bnd);
             HIPCHK(hipMemcpy(al.data(), a->bc_alpha, sizeof(double) * a->n_bnd, hipMemcpyDeviceToHost));
             HIPCHK(hipMemcpy(be.data(), a->bc_beta, sizeof(double) * a->n_bnd, hipMemcpyDeviceToHost));
@@ -250,12 +239,23 @@ int rbfx_build_weights(const rbfx_build_args* a)
 
     std::lock_guard<std::mutex> lk(g_mu);
     DeviceState& ds = dev_state();
+    // an asynchronous (capturable) build allocates nothing: the library's lists must
+    // already have room for n_eval stencils from an earlier blocking build
+    const size_t need = sizeof(int32_t) * (size_t)a->n_eval;
+    if (async) {
+        if (!a->status && ds.cap_status < need)
+            return set_err(RBFX_EINVAL, "asynchronous build needs args.status or an earlier blocking build");
+        if (a->precision != RBFX_PREC_FP64 && (ds.cap_fail < need || !ds.fail_count))
+            return set_err(RBFX_EINVAL, "asynchronous build: run one blocking build with these sizes first");
+        if (a->precision == RBFX_PREC_LEARNED && (ds.cap_reject < need || !ds.reject_count))
+            return set_err(RBFX_EINVAL, "asynchronous build: run one blocking build with these sizes first");
+    }
     int32_t* status = a->status;
     if (!status) {
-        HIPCHK(grow((void**)&ds.status, ds.cap_status, sizeof(int32_t) * (size_t)a->n_eval));
+        if (!async) HIPCHK(grow((void**)&ds.status, ds.cap_status, need));
         status = ds.status;
     }
-    if (!ds.hist) HIPCHK(hipMalloc((void**)&ds.hist, 16 * sizeof(unsigned long long)));
+    if (!ds.hist && !async) HIPCHK(hipMalloc((void**)&ds.hist, 16 * sizeof(unsigned long long)));
 
     rbfx::BuildDev P;
     std::memset(&P, 0, sizeof(P));
@@ -295,25 +295,36 @@ int rbfx_build_weights(const rbfx_build_args* a)
     P.ok_status = RBFX_ST_OK;
 
     const int nmax = a->k + M;
-    for (int i = 0; i < 4; ++i)
-        if (!ds.ev[i]) HIPCHK(hipEventCreate(&ds.ev[i]));
     // stage boundaries: ev0 | main kernel | ev1 | FP32 list solve | ev2 | FP64 fallback | ev3
-    HIPCHK(hipEventRecord(ds.ev[0], st));
+    // (blocking builds only; an asynchronous build records nothing but kernels)
+    auto mark = [&](int i) -> hipError_t {
+        if (async) return hipSuccess;
+        if (!ds.ev[i]) {
+            hipError_t e = hipEventCreate(&ds.ev[i]);
+            if (e != hipSuccess) return e;
+        }
+        return hipEventRecord(ds.ev[i], st);
+    };
+    HIPCHK(mark(0));
     if (a->precision == RBFX_PREC_FP64) {
         HIPCHK(rbfx::launch_build(P, a->dim, nmax, false, a->n_eval, st));
-        HIPCHK(hipEventRecord(ds.ev[1], st));
-        HIPCHK(hipEventRecord(ds.ev[2], st));
+        HIPCHK(mark(1));
+        HIPCHK(mark(2));
     } else {
-        HIPCHK(grow((void**)&ds.fail_list, ds.cap_fail, sizeof(int32_t) * (size_t)a->n_eval));
-        if (!ds.fail_count) HIPCHK(hipMalloc((void**)&ds.fail_count, sizeof(int32_t)));
+        if (!async) {
+            HIPCHK(grow((void**)&ds.fail_list, ds.cap_fail, need));
+            if (!ds.fail_count) HIPCHK(hipMalloc((void**)&ds.fail_count, sizeof(int32_t)));
+        }
         hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
         P.fail_list = ds.fail_list;
         P.fail_count = ds.fail_count;
         int64_t n_solve = a->n_eval;
         if (a->precision == RBFX_PREC_LEARNED) {
             // surrogate + FP64 acceptance; the rejected stencils go to the solver
-            HIPCHK(grow((void**)&ds.reject_list, ds.cap_reject, sizeof(int32_t) * (size_t)a->n_eval));
-            if (!ds.reject_count) HIPCHK(hipMalloc((void**)&ds.reject_count, sizeof(int32_t)));
+            if (!async) {
+                HIPCHK(grow((void**)&ds.reject_list, ds.cap_reject, need));
+                if (!ds.reject_count) HIPCHK(hipMalloc((void**)&ds.reject_count, sizeof(int32_t)));
+            }
             hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.reject_count);
             rbfx::LearnedDev Q;
             std::memset(&Q, 0, sizeof(Q));
@@ -341,20 +352,22 @@ int rbfx_build_weights(const rbfx_build_args* a)
             Q.reject_count = ds.reject_count;
             Q.out_rho = mdl->out_rho;
             HIPCHK(rbfx::launch_learned_build(Q, st));
-            HIPCHK(hipEventRecord(ds.ev[1], st));
-            int32_t nrej = 0;
-            HIPCHK(read_count(ds.reject_count, &nrej, st));
+            HIPCHK(mark(1));
             P.list = ds.reject_list;
             P.list_len = ds.reject_count;
-            n_solve = nrej;
+            if (!async) {
+                int32_t nrej = 0;
+                HIPCHK(read_count(ds.reject_count, &nrej, st));
+                n_solve = nrej;
+            }  // asynchronous: the grid covers every stencil, the kernel reads the count
         }
         if (n_solve > 0) HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, n_solve, st));
         // FP32_REFINED: the FP32 solve is the main kernel (stage 1 stays empty)
-        if (a->precision != RBFX_PREC_LEARNED) HIPCHK(hipEventRecord(ds.ev[1], st));
-        HIPCHK(hipEventRecord(ds.ev[2], st));
+        if (a->precision != RBFX_PREC_LEARNED) HIPCHK(mark(1));
+        HIPCHK(mark(2));
         // FP64 re-solve of the stencils whose refinement did not converge
-        int32_t nfail = 0;
-        HIPCHK(read_count(ds.fail_count, &nfail, st));
+        int32_t nfail = (int32_t)a->n_eval;
+        if (!async) HIPCHK(read_count(ds.fail_count, &nfail, st));
         rbfx::BuildDev F = P;
         F.fail_list = nullptr;
         F.fail_count = nullptr;
@@ -363,7 +376,12 @@ int rbfx_build_weights(const rbfx_build_args* a)
         F.ok_status = RBFX_ST_FALLBACK64;
         if (nfail > 0) HIPCHK(rbfx::launch_build(F, a->dim, nmax, false, nfail, st));
     }
-    HIPCHK(hipEventRecord(ds.ev[3], st));
+    HIPCHK(mark(3));
+    if (async) {
+        HIPCHK(hipGetLastError());
+        g_err.clear();
+        return RBFX_OK;
+    }
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, ds.hist, 16);
     const int64_t hb = std::min<int64_t>((a->n_eval + 255) / 256, 4096);
     hipLaunchKernelGGL(k_hist16, dim3((unsigned)hb), dim3(256), 0, st, status, a->n_eval, ds.hist);
@@ -391,6 +409,59 @@ int rbfx_build_weights(const rbfx_build_args* a)
     g_err.clear();
     return RBFX_OK;
 }
+
+
+extern "C" {
+
+int32_t rbfx_abi_version(void) { return 2; }
+
+const char* rbfx_last_error(void) { return g_err.c_str(); }
+
+int rbfx_init(int32_t device)
+{
+    int n = 0;
+    HIPCHK(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return set_err(RBFX_EINVAL, "device %d out of range (%d devices)", device, n);
+    HIPCHK(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return set_err(RBFX_EDEVICE, "rbfx kernels are built for gfx950; device %d is %s", device, prop.gcnArchName);
+    g_err.clear();
+    return RBFX_OK;
+}
+
+int rbfx_last_build_counts(int64_t counts[16])
+{
+    for (int i = 0; i < 16; ++i) counts[i] = g_counts[i];
+    return RBFX_OK;
+}
+
+int rbfx_last_build_timing(double ms[4])
+{
+    for (int i = 0; i < 4; ++i) ms[i] = g_stage_ms[i];
+    return RBFX_OK;
+}
+
+int rbfx_knn(const double* data, int64_t n_data, const double* query, int64_t n_query, int32_t dim, int32_t k,
+             const int64_t* q_ptr, int32_t* out_idx, void* stream)
+{
+    if (!data || !query || !out_idx) return set_err(RBFX_EINVAL, "rbfx_knn: null pointer argument");
+    if (dim < 1 || dim > 3) return set_err(RBFX_EINVAL, "rbfx_knn: dim must be 1, 2 or 3 (got %d)", dim);
+    if (k < 1 || k > n_data) return set_err(RBFX_EINVAL, "rbfx_knn: need 1 <= k <= n_data (k=%d, n_data=%lld)", k, (long long)n_data);
+    if (k > 64) return set_err(RBFX_EUNSUPPORTED, "rbfx_knn: k=%d exceeds the supported maximum of 64", k);
+    if (n_query == 0) return RBFX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_mu);
+    DeviceState& ds = dev_state();
+    HIPCHK(rbfx::run_knn(ds.knn, data, n_data, query, n_query, dim, k, q_ptr, out_idx, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+int rbfx_build_weights(const rbfx_build_args* a) { return build_impl(a, false); }
+
+int rbfx_build_weights_async(const rbfx_build_args* a) { return build_impl(a, true); }
 
 int64_t rbfx_mlp_param_count(void) { return rbfx::mlp_n_params(); }
 
@@ -570,40 +641,8 @@ int rbfx_spatial_order(const double* points, int64_t n, int32_t dim, int32_t* ou
     return RBFX_OK;
 }
 
-int rbfx_spmv(const rbfx_spmv_args* a)
-{
-    if (!a) return set_err(RBFX_EINVAL, "rbfx_spmv: null args");
-    if (!a->col_idx || !a->vals || !a->x || !a->y) return set_err(RBFX_EINVAL, "rbfx_spmv: null pointer argument");
-    if (a->n_fields < 1 || a->n_fields > 64) return set_err(RBFX_EUNSUPPORTED, "rbfx_spmv: n_fields=%d out of range 1..64", a->n_fields);
-    if (a->n_valarrays < 1) return set_err(RBFX_EINVAL, "rbfx_spmv: n_valarrays must be >= 1");
-    if (!a->row_ptr && a->k < 1) return set_err(RBFX_EINVAL, "rbfx_spmv: k must be >= 1 without row_ptr");
-    if (a->n_rows == 0) return RBFX_OK;
-    hipStream_t st = (hipStream_t)a->stream;
-    rbfx::SpmvDev P;
-    P.n_rows = a->n_rows;
-    P.row_ptr = a->row_ptr;
-    P.k = a->k;
-    P.col_idx = a->col_idx;
-    P.nnz = a->nnz > 0 ? a->nnz : a->n_rows * (int64_t)a->k;
-    P.n_fields = a->n_fields;
-    P.x = a->x;
-    P.x_ld_point = a->x_ld_point;
-    P.x_ld_field = a->x_ld_field;
-    P.y_ld_point = a->y_ld_point;
-    P.y_ld_field = a->y_ld_field;
-    P.y_ld_val = a->y_ld_val;
-    P.alpha = a->alpha;
-    P.beta = a->beta;
-    P.row_order = a->row_order;
-    for (int v0 = 0; v0 < a->n_valarrays; v0 += 4) {
-        const int nv = std::min(4, a->n_valarrays - v0);
-        P.n_valarrays = nv;
-        P.vals = a->vals + (int64_t)v0 * P.nnz;
-        P.y = a->y + (int64_t)v0 * a->y_ld_val;
-        HIPCHK(rbfx::launch_spmv(P, st));
-    }
-    HIPCHK(hipStreamSynchronize(st));
-    return RBFX_OK;
-}
+int rbfx_spmv(const rbfx_spmv_args* a) { return spmv_impl(a, false); }
+
+int rbfx_spmv_async(const rbfx_spmv_args* a) { return spmv_impl(a, true); }
 
 }  // extern "C"

@@ -443,9 +443,20 @@ class RadialBasisOperator:
                 np.cumsum(lens, out=ptr[1:])
                 row_ptr = t.as_tensor(ptr, device=self.dev)
         nnz = int(row_ptr[-1].item()) if row_ptr is not None else self.n_eval * self.k
-        vals = t.empty((n_cols, nnz), dtype=t.float64, device=self.dev)
-        col = t.empty(nnz, dtype=t.int32, device=self.dev) if row_ptr is not None else None
-        status = t.empty(self.n_eval, dtype=t.int32, device=self.dev)
+        # output buffers are reused across rebuilds (a captured graph keeps their addresses)
+        old = getattr(self, "vals", None)
+        if old is not None and tuple(old.shape) == (n_cols, nnz):
+            vals = old
+        else:
+            vals = t.empty((n_cols, nnz), dtype=t.float64, device=self.dev)
+        col = None
+        if row_ptr is not None:
+            oc = getattr(self, "col", None)
+            col = oc if (oc is not None and oc.numel() == nnz and oc is not getattr(self, "_adj_flat", None)) \
+                else t.empty(nnz, dtype=t.int32, device=self.dev)
+        ost = getattr(self, "status", None)
+        status = ost if (ost is not None and ost.numel() == self.n_eval) \
+            else t.empty(self.n_eval, dtype=t.int32, device=self.dev)
         a.out_ptr = _lib.ptr(row_ptr)
         a.out_col = _lib.ptr(col)
         a.out_val = _lib.ptr(vals)
@@ -461,6 +472,7 @@ class RadialBasisOperator:
             mstruct = self.model.struct(self.model._rho)
             keep.append(mstruct)
             a.model = C.addressof(mstruct)
+        self._args, self._keep = a, keep
         rc, counts = _lib.build_weights(a, self.dev)
         self.status_counts = counts
         self.status = status
@@ -474,7 +486,8 @@ class RadialBasisOperator:
                 raise NotImplementedError(msg)
             raise ArgumentError(msg)
         self.row_ptr = row_ptr
-        self.col = col if col is not None else self.adj_idx.reshape(-1)
+        self._adj_flat = self.adj_idx.reshape(-1)
+        self.col = col if col is not None else self._adj_flat
         self.vals = vals
         self.n_cols = n_cols
         self._host_weights = None
@@ -483,6 +496,30 @@ class RadialBasisOperator:
         """update_weights!(op) (operators.jl:589-602)."""
         self._build()
         self.valid_cache = True
+
+    def _build_async(self):
+        """Re-enqueue the last build on the current stream (no sync, no read-back)."""
+        a = self._args
+        a.stream = _lib.stream_ptr(self.dev)
+        _lib.build_weights_async(a, self.dev)
+
+    def capture(self, x):
+        """Record weight build + apply of the field x (n_data,) or (n_data, F) in a HIP
+        graph. Returns a GraphedOperator whose replay() rebuilds the weights from the
+        current coordinates and applies them to its `x` buffer into its `y` buffer."""
+        t = _lib.torch()
+        xd = _to_device(x, self.dev, t.float64)
+        X = (xd[:, None] if xd.ndim == 1 else xd.reshape(xd.shape[0], -1)).clone().contiguous()
+        Y = t.empty((self.n_cols, self.n_eval, X.shape[1]), dtype=t.float64, device=self.dev)
+        self._build()               # blocking warm-up: sizes the library's lists
+        self._spmv(X, out=Y)        # warm-up: computes the row order
+        t.cuda.synchronize(self.dev)
+        g = t.cuda.CUDAGraph()
+        stream = t.cuda.Stream(self.dev)
+        with t.cuda.graph(g, stream=stream):
+            self._build_async()
+            self._spmv(X, out=Y, asynchronous=True)
+        return GraphedOperator(self, g, X, Y)
 
     def is_cache_valid(self):
         return self.valid_cache
@@ -506,7 +543,7 @@ class RadialBasisOperator:
         return self._host_weights
 
     # ---------------------------------------------------------------- apply
-    def _spmv(self, X, out=None, alpha=1.0, beta=0.0):
+    def _spmv(self, X, out=None, alpha=1.0, beta=0.0, asynchronous=False):
         """X: (n_data, F) float64 cuda, point-major. Returns (n_cols, n_eval, F)."""
         t = _lib.torch()
         F = X.shape[1]
@@ -536,7 +573,7 @@ class RadialBasisOperator:
             # neighbouring X entries (the result does not depend on the order)
             self._row_order = _lib.spatial_order(self._d_eval)
         s.row_order = _lib.ptr(self._row_order)
-        _lib.spmv(s, self.dev)
+        _lib.spmv(s, self.dev, asynchronous=asynchronous)
         return Y
 
     def __call__(self, x, y=None):
@@ -660,6 +697,19 @@ def is_cache_valid(op):
 
 
 # ============================================================== constructors
+class GraphedOperator:
+    """Build + apply of one operator recorded in a HIP graph (RadialBasisOperator.capture).
+    `x` is the input field buffer (write new values into it), `y` the output
+    (n_cols, n_eval, F); replay() re-runs the whole captured sequence."""
+
+    def __init__(self, op, graph, x, y):
+        self.op, self.graph, self.x, self.y = op, graph, x, y
+
+    def replay(self):
+        self.graph.replay()
+        return self.y
+
+
 def laplacian(data, basis=None, **kw):
     return RadialBasisOperator(Laplacian(), data, basis=basis, **kw)
 
