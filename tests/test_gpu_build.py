@@ -366,3 +366,162 @@ def test_graph_capture_build_and_apply(RBF, gpu, precision):
         assert torch.equal(op.vals, w_eager), rep
         y_ref = op(torch.as_tensor(f, device=gpu))
         assert torch.equal(y[0, :, 0], y_ref), rep
+
+
+# ------------------------------------------------ config-shaped cases (reduced N)
+def _raw_build(RBF, gpu, pts, adj, basis, n_cols, terms, precision=0):
+    """One build through the C ABI with an arbitrary term table (fused RHS columns)."""
+    import torch
+
+    A = RBF._abi
+    d = torch.as_tensor(pts, device=gpu)
+    aj = torch.as_tensor(adj.astype(np.int32), device=gpu)
+    n, k = adj.shape
+    vals = torch.empty(n_cols * n * k, dtype=torch.float64, device=gpu)
+    a = A.BuildArgs()
+    a.dim = pts.shape[1]
+    a.k = k
+    a.n_data = len(pts)
+    a.n_eval = n
+    a.data = d.data_ptr()
+    a.eval = d.data_ptr()
+    a.adj_idx = aj.data_ptr()
+    a.basis = basis
+    a.op = A.make_operator(n_cols, terms)
+    a.out_val = vals.data_ptr()
+    a.out_nnz = n * k
+    a.precision = precision
+    a.stream = RBF._lib.stream_ptr(gpu).value
+    rc, counts = RBF._lib.build_weights(a, gpu)
+    assert rc == 0, RBF._lib.last_error()
+    return vals.reshape(n_cols, n * k).cpu().numpy()
+
+
+def test_c2_shape_3d_gradient_laplacian(RBF, O, gpu):
+    """3-D jittered grid, PHS(5), degree 3, k=40 (n=60), gradient + Laplacian fused
+    into one build (4 RHS columns, one factorisation); weights against the
+    extended-precision oracle, the fused columns against separate builds, and the
+    applied operators against the analytic derivatives of exp(x) sin(2y) cos(3z)."""
+    m = 16
+    h = 1.0 / (m - 1)
+    g = np.stack(np.meshgrid(*[np.linspace(0, 1, m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    rng = np.random.default_rng(2)
+    pts = g + rng.uniform(-0.3 * h, 0.3 * h, g.shape)
+    op = RBF.RadialBasisOperator(RBF.Jacobian(), pts, basis=RBF.PHS(5, poly_deg=3), k=40, device=gpu)
+    lap = RBF.RadialBasisOperator(RBF.Laplacian(), pts, basis=RBF.PHS(5, poly_deg=3), k=40, device=gpu,
+                                  adjl=op.adj_idx)
+    adj = op.adj_idx.cpu().numpy()
+    terms = [(c, O.abi.OP_D1, c, 0, 1.0) for c in range(3)] + [(3, O.abi.OP_LAP, 0, 0, 1.0)]
+    fused = _raw_build(RBF, gpu, pts, adj, RBF._abi.Basis(5, 3, 0.0), 4, terms)
+    wg = np.concatenate([op.vals.cpu().numpy(), lap.vals.cpu().numpy()], 0)
+    assert np.abs(fused - wg).max() <= 1e-12 * np.abs(wg).max()
+    sel = rng.choice(len(pts), 600, replace=False)
+    wsel = fused.reshape(4, -1, 40)[:, sel].reshape(4, -1)
+    check_against_truth(O, wsel, pts, pts[sel], adj[sel], O.basis_struct(5, 3), _op(O, 4, terms), 40)
+    f = np.exp(pts[:, 0]) * np.sin(2 * pts[:, 1]) * np.cos(3 * pts[:, 2])
+    gy = 2 * np.exp(pts[:, 0]) * np.cos(2 * pts[:, 1]) * np.cos(3 * pts[:, 2])
+    gz = -3 * np.exp(pts[:, 0]) * np.sin(2 * pts[:, 1]) * np.sin(3 * pts[:, 2])
+    inner = np.all((pts > 0.15) & (pts < 0.85), 1)
+    G = op(f)
+    for c, ex in enumerate((f, gy, gz)):
+        e = np.abs(G[:, c] - ex)[inner].max() / np.abs(ex[inner]).max()
+        assert e < 1e-3, (c, e)
+    e = np.abs(lap(f) + 12 * f)[inner].max() / np.abs(12 * f[inner]).max()
+    assert e < 5e-3, e
+
+
+def test_c3_shape_disk_hermite_variable_k(RBF, O, gpu):
+    """Unit disk: Gaussian-mixture interior + uniformly spaced boundary nodes with
+    Neumann rows, k adapted to local density in [12, 54], PHS(3) degree 3 (n <= 64),
+    partial dx + d2y, FP32 factorisation with FP64 refinement; weights against the
+    extended-precision oracle for a sample of stencils (interior and Hermite)."""
+    rng = np.random.default_rng(33)
+    cen = rng.uniform(-0.6, 0.6, (8, 2))
+    sig = rng.uniform(0.02, 0.2, 8)
+    pts = []
+    while len(pts) < 6000:
+        j = rng.integers(8)
+        p = cen[j] + sig[j] * rng.standard_normal(2)
+        if p @ p < 0.97:
+            pts.append(p)
+    nb = 400
+    th = 2 * np.pi * np.arange(nb) / nb
+    bnd = np.stack([np.cos(th), np.sin(th)], 1)
+    allp = np.concatenate([np.array(pts), bnd])
+    isb = np.zeros(len(allp), dtype=bool)
+    isb[-nb:] = True
+    # k from local density: distance to the 12th neighbour against the median
+    from scipy.spatial import cKDTree
+
+    d12 = cKDTree(allp).query(allp, 13)[0][:, -1]
+    kk = np.clip((24 * (np.median(d12) / d12) ** 2).astype(int), 12, 54)
+    tree = cKDTree(allp)
+    adjl = [tree.query(allp[i], kk[i])[1].astype(np.int32) for i in range(len(allp))]
+    herm = RBF.Hermite(isb, [RBF.Neumann()] * nb, bnd)
+    L = RBF.Partial(1, 1) + RBF.Partial(2, 2)
+    op = RBF.RadialBasisOperator(L, allp, basis=RBF.PHS(3, poly_deg=3), adjl=adjl, hermite=herm, device=gpu,
+                                 precision="fp32_refined")
+    ptr = op.row_ptr.cpu().numpy()
+    vals = op.vals.cpu().numpy()[0]
+    sel = np.concatenate([rng.choice(len(pts), 200, replace=False), len(pts) + np.arange(0, nb, 10)])
+    opk = _op(O, 1, [(0, O.abi.OP_D1, 0, 0, 1.0), (0, O.abi.OP_D2, 1, 0, 1.0)])
+    rows = []
+    for i in sel:
+        rows.append(vals[ptr[i]:ptr[i + 1]])
+    sel_adj = [adjl[i] for i in sel]
+    sptr = np.zeros(len(sel) + 1, dtype=np.int64)
+    np.cumsum([len(a) for a in sel_adj], out=sptr[1:])
+    sidx = np.concatenate(sel_adj).astype(np.int32)
+    _, _, wld, _, rc = O.build(allp, allp[sel], (sptr, sidx), O.basis_struct(3, 3), opk, scaled=True,
+                               long_double=True, hermite=(isb, np.zeros(nb), np.ones(nb), bnd))
+    _, _, w64, _, rc2 = O.build(allp, allp[sel], (sptr, sidx), O.basis_struct(3, 3), opk, scaled=True,
+                                hermite=(isb, np.zeros(nb), np.ones(nb), bnd))
+    assert rc == 0 and rc2 == 0
+    for j in range(len(sel)):
+        g = rows[j]
+        t = wld[0][sptr[j]:sptr[j + 1]]
+        f64 = w64[0][sptr[j]:sptr[j + 1]]
+        eg = np.abs(g - t).max() / np.abs(t).max()
+        ef = np.abs(f64 - t).max() / np.abs(t).max()
+        assert eg <= max(1e-10, 10 * ef), (j, sel[j], eg, ef)
+    st = op.status_counts
+    assert st[RBF._abi.ST_REFINED] + st[RBF._abi.ST_FALLBACK64] == len(allp)
+
+
+def test_c5_shape_regrid_imq_multifield(RBF, O, gpu):
+    """3-D regrid: uniform sources to a jittered target grid, IMQ eps=3, degree 1,
+    k=30, FP32 build + FP64 refinement, 16 fields applied at once; the applied values
+    match the oracle weights applied on the CPU, and interpolation of smooth Fourier
+    fields is accurate."""
+    import torch
+
+    rng = np.random.default_rng(55)
+    src = rng.random((20000, 3))
+    m = 22
+    h = 1.0 / (m - 1)
+    tg = np.stack(np.meshgrid(*[np.linspace(0, 1, m)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    tg = np.clip(tg + rng.uniform(-0.3 * h, 0.3 * h, tg.shape), 0, 1)
+    op = RBF.regrid(src, tg, basis=RBF.IMQ(3.0, poly_deg=1), k=30, device=gpu, precision="fp32_refined")
+    kap = rng.integers(-6, 7, (16, 8, 3))
+    ph = rng.uniform(0, 2 * np.pi, (16, 8))
+    amp = rng.standard_normal((16, 8))
+
+    def fields(x):
+        return np.stack([(amp[f] * np.cos(2 * np.pi * 0.25 * (x @ kap[f].T) + ph[f])).sum(1) for f in range(16)], 1)
+
+    Fs, Ft = fields(src), fields(tg)
+    Y = op(torch.as_tensor(Fs, device=gpu)).cpu().numpy()
+    W = op.weights()
+    ref = W @ Fs
+    assert np.allclose(Y, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    adj = op.adj_idx.cpu().numpy()
+    sel = rng.choice(len(tg), 400, replace=False)
+    _, _, wld, _, rc = O.build(src, tg[sel], adj[sel], O.basis_struct(O.abi.RBFX_IMQ, 1, 3.0),
+                               _op(O, 1, [(0, O.abi.OP_VALUE, 0, 0, 1.0)]), scaled=True, long_double=True)
+    assert rc == 0
+    wg = op.vals.cpu().numpy()[0].reshape(-1, 30)[sel]
+    yg = (wg * Fs[adj[sel]].transpose(2, 0, 1)).sum(2).T
+    yl = (wld[0].reshape(-1, 30) * Fs[adj[sel]].transpose(2, 0, 1)).sum(2).T
+    assert np.abs(yg - yl).max() <= 1e-9 * np.abs(yl).max()
+    err = np.sqrt(np.mean((Y - Ft) ** 2)) / np.sqrt(np.mean(Ft ** 2))
+    assert err < 5e-2, err
