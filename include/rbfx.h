@@ -156,8 +156,9 @@ typedef struct rbfx_build_args {
 } rbfx_build_args;
 
 /* Surrogate for RBFX_PREC_LEARNED (no reference counterpart). A BF16 MLP
- * (2k -> 256 -> 256 -> 256 -> k+m, GELU, FP32 accumulation) maps the canonical,
- * radius-scaled stencil offsets to (w~, lambda) of the scaled system. The host
+ * (4k -> 256 -> 256 -> 256 -> k+m, GELU, FP32 accumulation) maps the canonical,
+ * radius-scaled stencil offsets to (w~, lambda) r1^2 of the scaled system (r1: the
+ * distance of the canonical reference neighbour). The host
  * struct holds device pointers; the signature fields must match the build.     */
 typedef struct rbfx_mlp_model {
     int32_t dim;                 /* 2                                                  */
@@ -179,7 +180,7 @@ typedef struct rbfx_mlp_model {
 typedef struct rbfx_mlp_run_args {
     int32_t batch;               /* M stencils                                         */
     int32_t n_out;               /* outputs used by the loss (k + m <= 32)             */
-    const uint16_t* x;           /* M x 64 BF16 inputs (canonical offsets, zero pad)   */
+    const uint16_t* x;           /* M x 128 BF16 inputs (rbfx_stencil_features)        */
     const float* target;         /* M x n_out standardised targets, or NULL            */
     const float* params;
     const uint16_t* wbf16;
@@ -188,6 +189,7 @@ typedef struct rbfx_mlp_run_args {
     float* y_out;                /* optional M x 32 FP32 network outputs                */
     void* workspace;             /* rbfx_mlp_workspace_bytes(batch) bytes               */
     void* stream;
+    const float* row_weight;     /* optional M per-stencil loss weights (1 if NULL)    */
 } rbfx_mlp_run_args;
 
 /* Arguments of rbfx_spmv: Y_v = alpha * W_v X + beta * Y_v for every value array v.
@@ -216,6 +218,15 @@ typedef struct rbfx_spmv_args {
                                     processed (e.g. from rbfx_spatial_order) so the X
                                     gathers of consecutive rows share cache lines; the
                                     result does not depend on it                        */
+    int32_t rows_permuted;       /* ABI 2: nonzero when row_ptr / col_idx / vals are
+                                    stored in row_order (rbfx_permute_rows)             */
+    int64_t max_row_len;         /* ABI 2, optional: longest row (0 = unknown; k is used
+                                    without row_ptr). Lets a workgroup stage its rows'
+                                    entries through LDS; an underestimate is safe (those
+                                    workgroups read HBM directly)                       */
+    int32_t mapping;             /* ABI 2: 0 automatic, 1 lanes per row reading HBM,
+                                    2 staged through LDS (needs rows stored in processing
+                                    order and F < 8; RBFX_EUNSUPPORTED otherwise)        */
 } rbfx_spmv_args;
 
 /* ---- entry points -------------------------------------------------------- */
@@ -260,6 +271,13 @@ int rbfx_spmv_async(const rbfx_spmv_args* args);
  * moving user data.                                                               */
 int rbfx_spatial_order(const double* points, int64_t n, int32_t dim, int32_t* out_order, void* stream);
 
+/* Apply plan for fixed-length rows (no reference counterpart): out row r = in row
+ * row_order[r], for the column indices and every value array ([n_valarrays][nnz]),
+ * so an apply with rows_permuted streams its weights while walking row_order.     */
+int rbfx_permute_rows(int64_t n_rows, int32_t k, const int32_t* row_order, const int32_t* col_idx,
+                      const double* vals, int64_t nnz, int32_t n_valarrays, int32_t* out_col, double* out_vals,
+                      void* stream);
+
 /* Number of stencils per status code from the last rbfx_build_weights on this
  * thread (counts[0..15]); lets the host raise the reference's exception types. */
 int rbfx_last_build_counts(int64_t counts[16]);
@@ -292,10 +310,12 @@ int rbfx_mlp_adam(float* params, const float* grads, float* m, float* v, uint16_
 int rbfx_mlp_sync_bf16(const float* params, uint16_t* wbf16, void* stream);
 
 /* Canonical radius-scaled offsets of fixed-size 2-D stencils: out_xi (n_eval*k*2
- * FP64, may be NULL), the network inputs out_x (n_eval*64 BF16, may be NULL) and
- * the stencil radius out_h (n_eval, may be NULL).                                */
+ * FP64), the network inputs out_x (n_eval*128 BF16: the 2k offsets' BF16 high parts,
+ * their BF16 remainders, zeros), the stencil radius out_h and the distance of the
+ * canonical reference neighbour out_r1 (n_eval each). Any output may be NULL.    */
 int rbfx_stencil_features(const double* data, const double* eval, int64_t n_eval, int32_t dim, int32_t k,
-                          const int32_t* adj_idx, double* out_xi, uint16_t* out_x, double* out_h, void* stream);
+                          const int32_t* adj_idx, double* out_xi, uint16_t* out_x, double* out_h, double* out_r1,
+                          void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,12 +12,12 @@ The network has no counterpart in the reference package; it is pinned by
   - the closed form of the tanh-GELU derivative, checked against differences.
 
 Parameter layout (flat float32 array): per layer l, Wt_l [N_l x K_l] then b_l [N_l],
-with (K, N) = (64, 256), (256, 256), (256, 256), (256, 32).
+with (K, N) = (128, 256), (256, 256), (256, 256), (256, 32).
 """
 import numpy as np
 
 LAYERS = 4
-IN_PAD, HID, OUT_PAD = 64, 256, 32
+IN_PAD, HID, OUT_PAD = 128, 256, 32
 C_GELU = np.sqrt(2.0 / np.pi)
 
 
@@ -67,7 +67,7 @@ def unpack(params):
 
 
 def forward(params, X, bf16_round=True):
-    """X: (M, 64) inputs. Returns (Y (M, 32), cache)."""
+    """X: (M, 128) inputs. Returns (Y (M, 32), cache)."""
     rnd = bf16 if bf16_round else (lambda a: np.asarray(a, dtype=np.float64))
     Ws, bs = unpack(params)
     Wq = [rnd(W) for W in Ws]
@@ -81,15 +81,17 @@ def forward(params, X, bf16_round=True):
     return Zs[-1], dict(Hs=Hs, Zs=Zs, Wq=Wq)
 
 
-def loss_and_grad(params, X, T, n_out, bf16_round=True):
-    """Mean squared error over the first n_out outputs and its gradient (flat)."""
+def loss_and_grad(params, X, T, n_out, bf16_round=True, row_weight=None):
+    """Row-weighted mean squared error over the first n_out outputs,
+    mean_m w_m mean_n (Y - T)^2, and its gradient (flat)."""
     rnd = bf16 if bf16_round else (lambda a: np.asarray(a, dtype=np.float64))
     Y, c = forward(params, X, bf16_round)
     M = X.shape[0]
+    w = np.ones(M) if row_weight is None else np.asarray(row_weight, dtype=np.float64)
     D = np.zeros_like(Y)
     D[:, :n_out] = Y[:, :n_out] - T[:, :n_out]
-    loss = float(np.mean(D[:, :n_out] ** 2))
-    dZ = rnd(2.0 * D / (M * n_out))
+    loss = float(np.mean(w[:, None] * D[:, :n_out] ** 2))
+    dZ = rnd(2.0 * w[:, None] * D / (M * n_out))
     g = np.zeros(N_PARAMS)
     for l in range(LAYERS - 1, -1, -1):
         Hin = c["Hs"][l]
@@ -109,10 +111,24 @@ def adam(p, g, m, v, lr, b1, b2, eps, t):
     return p - lr * mh / (np.sqrt(vh) + eps), m, v
 
 
+def network_inputs(xi):
+    """(n, k, 2) canonical offsets -> (n, 128) inputs: BF16 high parts of the 2k
+    offsets, then the BF16 roundings of their remainders, then zeros."""
+    n, k, _ = xi.shape
+    flat = xi.reshape(n, 2 * k)
+    hi = bf16(flat)
+    lo = bf16(flat - hi)
+    X = np.zeros((n, IN_PAD))
+    X[:, :2 * k] = hi
+    X[:, 2 * k:4 * k] = lo
+    return X
+
+
 def canonical_offsets(data, eval_points, adj):
     """Canonical radius-scaled offsets of 2-D stencils (the frame the network sees):
     xi = (x_i - x_e)/h, rotated so the first nonzero offset lies on +x, reflected
-    so the rotated y offsets sum to >= 0 (quad-partial summation order as the kernel)."""
+    so the rotated y offsets sum to >= 0 (quad-partial summation order as the kernel).
+    Returns (xi, h, r1) with r1 the scaled distance of that first nonzero offset."""
     D = data[adj] - eval_points[:, None, :]
     n, k, _ = D.shape
     h = np.sqrt((D ** 2).sum(2).max(1))
@@ -129,4 +145,4 @@ def canonical_offsets(data, eval_points, adj):
     part = [yr[:, q::4].sum(1) for q in range(4)]
     tot = (part[0] + part[1]) + (part[2] + part[3])
     yr = np.where((tot < 0)[:, None], -yr, yr)
-    return np.stack([xr, yr], 2), h
+    return np.stack([xr, yr], 2), h, r

@@ -56,6 +56,8 @@ def lib():
         L.orc_sym_solve_ld.argtypes = [C.c_int, C.c_int, dp, dp, dp]
         L.orc_build.restype = C.c_int
         L.orc_build.argtypes = [C.POINTER(abi.BuildArgs), C.c_int, C.c_int, C.c_int]
+        L.orc_assemble.restype = C.c_int
+        L.orc_assemble.argtypes = [C.POINTER(abi.BuildArgs), C.c_int64, C.c_int, dp]
         L.orc_knn.restype = C.c_int
         L.orc_knn.argtypes = [dp, C.c_int64, dp, C.c_int64, C.c_int, C.c_int,
                               C.POINTER(C.c_int32), C.c_int]
@@ -180,7 +182,7 @@ def hermite_arrays(n_data, is_boundary, alpha, beta, normals):
 
 
 def build(data, eval_points, adj, basis, op, hermite=None, scaled=False, long_double=False,
-          threads=0):
+          threads=0, _want_args=False):
     """Run the oracle build. adj: (n_eval, k) int array or (adj_ptr, adj_idx).
     op: abi.Operator. hermite: (is_boundary, alpha, beta, normals) per boundary point.
     Returns (out_ptr, out_col, out_val[n_cols, nnz], status, rc)."""
@@ -224,8 +226,27 @@ def build(data, eval_points, adj, basis, op, hermite=None, scaled=False, long_do
     a.out_val = _vp(out_val)
     a.out_nnz = nnz
     a.status = _vp(status)
+    if _want_args:
+        return a, keep + [out_ptr, out_col, out_val, status]
     rc = lib().orc_build(C.byref(a), int(scaled), int(long_double), int(threads))
     return out_ptr, out_col, out_val, status, rc
+
+
+def condition_numbers(data, eval_points, adj, basis, op, hermite=None, scaled=True):
+    """2-norm condition number of each stencil's assembled saddle-point system (the
+    matrix the solver factorises, in the radius-scaled frame by default); NaN for
+    identity (Dirichlet) rows. Used to state per-stencil parity bounds."""
+    a, keep = build(data, eval_points, adj, basis, op, hermite=hermite, scaled=scaled, _want_args=True)
+    n_eval = a.n_eval
+    m = lib().orc_n_monomials(a.dim, basis.poly_deg)
+    nmax = (int(a.k) if not a.adj_ptr else int(np.max(np.diff(keep[3])))) + m
+    buf = np.zeros(nmax * nmax)
+    out = np.full(n_eval, np.nan)
+    for e in range(n_eval):
+        n = lib().orc_assemble(C.byref(a), e, int(scaled), _dp(buf))
+        if n > 0:
+            out[e] = np.linalg.cond(buf[:n * n].reshape(n, n))
+    return out
 
 
 def spmv(row_ptr, col, val, x, threads=0):

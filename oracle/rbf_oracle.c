@@ -662,6 +662,7 @@ typedef struct {
     double* X;   /* n*r */
     double* xs;  /* k*dim local coords */
     pinfo* pinf; /* k */
+    double* dumpA; /* optional: receives the assembled n*n matrix before the solve */
 } workspace;
 
 /* one stencil: returns status code */
@@ -802,6 +803,7 @@ static int build_stencil(const rbfx_build_args* a, int64_t e, int scaled, int us
         }
     }
 
+    if (ws->dumpA) memcpy(ws->dumpA, A, sizeof(double) * n * n);
     int rc;
     if (use_ld) rc = orc_sym_solve_ld(n, r, A, B, ws->X);
     else rc = orc_sym_solve(n, r, A, B, ws->X);
@@ -857,6 +859,7 @@ int orc_build(const rbfx_build_args* a, int scaled, int long_double, int n_threa
         ws.X = (double*)malloc(sizeof(double) * nmax * r);
         ws.xs = (double*)malloc(sizeof(double) * kmax * dim + 8);
         ws.pinf = (pinfo*)malloc(sizeof(pinfo) * (kmax + 1));
+        ws.dumpA = NULL;
         double* w = (double*)malloc(sizeof(double) * kmax * r + 8);
 #ifdef _OPENMP
 #pragma omp for schedule(dynamic, 64)
@@ -888,6 +891,30 @@ int orc_build(const rbfx_build_args* a, int scaled, int long_double, int n_threa
         free(ws.A); free(ws.B); free(ws.X); free(ws.xs); free(ws.pinf); free(w);
     }
     return err;
+}
+
+/* The assembled system of stencil e (the matrix build_stencil factorises), for
+ * conditioning diagnostics: writes n*n doubles to A_out, returns n, or 0 for an
+ * identity (Dirichlet) row, or -1 on error.                                        */
+int orc_assemble(const rbfx_build_args* a, int64_t e, int scaled, double* A_out)
+{
+    const int m = orc_n_monomials(a->dim, a->basis.poly_deg);
+    const int r = a->op.n_cols;
+    int k = a->adj_ptr ? (int)(a->adj_ptr[e + 1] - a->adj_ptr[e]) : a->k;
+    const int n = k + m;
+    if (n > 128) return -1;
+    workspace ws;
+    ws.A = (double*)malloc(sizeof(double) * n * n);
+    ws.B = (double*)malloc(sizeof(double) * n * r);
+    ws.X = (double*)malloc(sizeof(double) * n * r);
+    ws.xs = (double*)malloc(sizeof(double) * k * a->dim + 8);
+    ws.pinf = (pinfo*)malloc(sizeof(pinfo) * (k + 1));
+    ws.dumpA = A_out;
+    double* w = (double*)malloc(sizeof(double) * k * r + 8);
+    int st = build_stencil(a, e, scaled, 0, &ws, w);
+    free(ws.A); free(ws.B); free(ws.X); free(ws.xs); free(ws.pinf); free(w);
+    if (st == RBFX_ST_DIRICHLET) return 0;
+    return n;
 }
 
 /* ---------------------------------------------------------------------------

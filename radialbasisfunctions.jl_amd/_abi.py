@@ -50,7 +50,7 @@ ABI_VERSION = 2
 
 # surrogate network shape (must match csrc/mlp.h)
 MLP_LAYERS = 4
-MLP_IN_PAD = 64
+MLP_IN_PAD = 128
 MLP_HID = 256
 MLP_OUT_PAD = 32
 
@@ -161,6 +161,7 @@ class MlpRunArgs(C.Structure):
         ("y_out", C.c_void_p),
         ("workspace", C.c_void_p),
         ("stream", C.c_void_p),
+        ("row_weight", C.c_void_p),
     ]
 
 
@@ -186,6 +187,9 @@ class SpmvArgs(C.Structure):
         ("beta", C.c_double),
         ("stream", C.c_void_p),
         ("row_order", C.c_void_p),
+        ("rows_permuted", C.c_int32),
+        ("max_row_len", C.c_int64),
+        ("mapping", C.c_int32),
     ]
 
 

@@ -56,6 +56,9 @@ def lib():
                 L.rbfx_spmv_async.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_last_build_counts.restype = C.c_int
                 L.rbfx_last_build_counts.argtypes = [C.POINTER(C.c_int64)]
+                L.rbfx_permute_rows.restype = C.c_int
+                L.rbfx_permute_rows.argtypes = [C.c_int64, C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                                C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
                 L.rbfx_spatial_order.restype = C.c_int
                 L.rbfx_spatial_order.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
                 L.rbfx_last_build_timing.restype = C.c_int
@@ -73,7 +76,8 @@ def lib():
                 L.rbfx_mlp_sync_bf16.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
                 L.rbfx_stencil_features.restype = C.c_int
                 L.rbfx_stencil_features.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_int32, C.c_int32,
-                                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+                                                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                    C.c_void_p]
                 _LIB = L
     return _LIB
 
@@ -82,7 +86,8 @@ def exported_symbols():
     return ["rbfx_init", "rbfx_last_error", "rbfx_abi_version", "rbfx_knn", "rbfx_build_weights",
             "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
-            "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async"]
+            "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async",
+            "rbfx_permute_rows"]
 
 
 def _check(rc):
@@ -163,7 +168,8 @@ def last_error():
     return lib().rbfx_last_error().decode(errors="replace")
 
 
-def mlp_run(x, params, wbf16, workspace, batch, target=None, grads=None, loss=None, y_out=None, n_out=0):
+def mlp_run(x, params, wbf16, workspace, batch, target=None, grads=None, loss=None, y_out=None, n_out=0,
+            row_weight=None):
     """One forward (and with target, backward) pass of the surrogate (device tensors)."""
     t = torch()
     dev = x.device
@@ -179,6 +185,7 @@ def mlp_run(x, params, wbf16, workspace, batch, target=None, grads=None, loss=No
     a.y_out = None if y_out is None else y_out.data_ptr()
     a.workspace = workspace.data_ptr()
     a.stream = stream_ptr(dev).value
+    a.row_weight = None if row_weight is None else row_weight.data_ptr()
     with t.cuda.device(dev):
         _check(lib().rbfx_mlp_run(C.byref(a)))
 
@@ -203,17 +210,19 @@ def mlp_workspace_bytes(batch):
 
 
 def stencil_features(data, eval_points, adj, k, want_xi=True, want_x=True, want_h=True):
-    """Canonical radius-scaled offsets of fixed-size 2-D stencils (device tensors)."""
+    """Canonical radius-scaled offsets of fixed-size 2-D stencils (device tensors):
+    (xi (n,k,2) f64, network inputs (n,128) bf16, h (n,) f64, r1 (n,) f64)."""
     t = torch()
     dev = data.device
     n = eval_points.shape[0]
     xi = t.empty((n, k, 2), dtype=t.float64, device=dev) if want_xi else None
     x = t.empty((n, abi.MLP_IN_PAD), dtype=t.bfloat16, device=dev) if want_x else None
     h = t.empty(n, dtype=t.float64, device=dev) if want_h else None
+    r1 = t.empty(n, dtype=t.float64, device=dev)
     with t.cuda.device(dev):
         _check(lib().rbfx_stencil_features(ptr(data), ptr(eval_points), n, data.shape[1], int(k), ptr(adj), ptr(xi),
-                                           ptr(x), ptr(h), stream_ptr(dev)))
-    return xi, x, h
+                                           ptr(x), ptr(h), ptr(r1), stream_ptr(dev)))
+    return xi, x, h, r1
 
 
 def last_build_timing():
@@ -231,3 +240,17 @@ def spatial_order(points):
     with t.cuda.device(dev):
         _check(lib().rbfx_spatial_order(ptr(points), points.shape[0], points.shape[1], ptr(out), stream_ptr(dev)))
     return out
+
+
+def permute_rows(order, k, col, vals):
+    """Apply plan: CSR rows of fixed length k copied into `order` (device tensors).
+    vals is (n_valarrays, nnz). Returns (col_p, vals_p)."""
+    t = torch()
+    dev = col.device
+    n = order.shape[0]
+    col_p = t.empty_like(col)
+    vals_p = t.empty_like(vals)
+    with t.cuda.device(dev):
+        _check(lib().rbfx_permute_rows(n, int(k), ptr(order), ptr(col), ptr(vals), vals.shape[1], vals.shape[0],
+                                       ptr(col_p), ptr(vals_p), stream_ptr(dev)))
+    return col_p, vals_p

@@ -144,6 +144,21 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
     P.alpha = a->alpha;
     P.beta = a->beta;
     P.row_order = a->row_order;
+    P.rows_permuted = a->rows_permuted;
+    if (a->rows_permuted && !a->row_order) return set_err(RBFX_EINVAL, "rbfx_spmv: rows_permuted needs row_order");
+    if (a->mapping < 0 || a->mapping > 2) return set_err(RBFX_EINVAL, "rbfx_spmv: mapping must be 0, 1 or 2");
+    if (a->max_row_len < 0) return set_err(RBFX_EINVAL, "rbfx_spmv: max_row_len must be >= 0");
+    P.max_row = a->row_ptr ? a->max_row_len : a->k;
+    P.mapping = a->mapping;
+    if (a->mapping == 2) {
+        const int64_t L = P.max_row;
+        const int g = L <= 20 ? 4 : (L <= 40 ? 8 : 16);
+        const int64_t lds = (256 / g) * L * (8 * (int64_t)std::min(4, a->n_valarrays) + 4);
+        if ((a->row_order && !a->rows_permuted) || a->n_fields >= 8 || L <= 0 || lds > 48 * 1024)
+            return set_err(RBFX_EUNSUPPORTED,
+                           "rbfx_spmv: staged mapping needs rows stored in processing order, n_fields < 8 and "
+                           "a longest row that fits LDS (got %lld)", (long long)L);
+    }
     for (int v0 = 0; v0 < a->n_valarrays; v0 += 4) {
         const int nv = std::min(4, a->n_valarrays - v0);
         P.n_valarrays = nv;
@@ -547,7 +562,8 @@ int rbfx_mlp_run(const rbfx_mlp_run_args* a)
     if (a->target) {
         HIPCHK(launch_fill_f32(a->loss, 1, 0.0f, st));
         HIPCHK(launch_fill_f32(a->grads, mlp_n_params(), 0.0f, st));
-        HIPCHK(launch_mse_grad(M, a->n_out, Zout, kOutPad, a->target, a->n_out, w.dY, kOutPad, a->loss, st));
+        HIPCHK(launch_mse_grad(M, a->n_out, Zout, kOutPad, a->target, a->n_out, a->row_weight, w.dY, kOutPad,
+                               a->loss, st));
         const __bf16* dZ = w.dY;
         int64_t ld_dz = kOutPad;
         for (int l = kMlpLayers - 1; l >= 0; --l) {
@@ -620,14 +636,29 @@ int rbfx_mlp_sync_bf16(const float* params, uint16_t* wbf16, void* stream)
 }
 
 int rbfx_stencil_features(const double* data, const double* eval, int64_t n_eval, int32_t dim, int32_t k,
-                          const int32_t* adj_idx, double* out_xi, uint16_t* out_x, double* out_h, void* stream)
+                          const int32_t* adj_idx, double* out_xi, uint16_t* out_x, double* out_h, double* out_r1,
+                          void* stream)
 {
     if (!data || !eval || !adj_idx) return set_err(RBFX_EINVAL, "rbfx_stencil_features: null pointer argument");
     if (dim != 2) return set_err(RBFX_EUNSUPPORTED, "rbfx_stencil_features: dim 2 only");
-    if (k < 1 || 2 * k > rbfx::kInPad) return set_err(RBFX_EUNSUPPORTED, "rbfx_stencil_features: need 1 <= k <= 32");
+    if (k < 1 || 4 * k > rbfx::kInPad) return set_err(RBFX_EUNSUPPORTED, "rbfx_stencil_features: need 1 <= k <= 32");
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(rbfx::launch_stencil_features(data, eval, n_eval, k, adj_idx, out_xi, reinterpret_cast<__bf16*>(out_x),
-                                         out_h, st));
+                                         out_h, out_r1, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+int rbfx_permute_rows(int64_t n_rows, int32_t k, const int32_t* row_order, const int32_t* col_idx,
+                      const double* vals, int64_t nnz, int32_t n_valarrays, int32_t* out_col, double* out_vals,
+                      void* stream)
+{
+    if (!row_order || !col_idx || !vals || !out_col || !out_vals)
+        return set_err(RBFX_EINVAL, "rbfx_permute_rows: null pointer argument");
+    if (k < 1 || n_valarrays < 1 || nnz < n_rows * (int64_t)k)
+        return set_err(RBFX_EINVAL, "rbfx_permute_rows: need k >= 1, n_valarrays >= 1, nnz >= n_rows*k");
+    hipStream_t st = (hipStream_t)stream;
+    HIPCHK(rbfx::launch_permute_rows(n_rows, k, row_order, col_idx, vals, nnz, n_valarrays, out_col, out_vals, st));
     HIPCHK(hipStreamSynchronize(st));
     return RBFX_OK;
 }

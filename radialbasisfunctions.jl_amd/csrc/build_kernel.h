@@ -21,7 +21,8 @@
 // Factorisation: symmetric-indefinite LDL^T with rook pivoting (the pivot tests of
 // Ashcraft, Grimes & Lewis 1998, as LAPACK ?sytrf_rook uses). Rows never move between
 // lanes. Column queries read each lane's own register (A is symmetric), a group
-// argmax (DPP rotations inside 16-lane rows, one cross-row exchange) picks the largest,
+// argmax over packed 32-bit (magnitude, lane) keys (DPP rotations inside 16-lane rows,
+// one cross-row exchange, one unsigned max per level) picks the largest,
 // and the pivot row(s) go through LDS as the broadcast source of the rank-1/rank-2
 // update. A lane's registers freeze when its row pivots, so they keep its row of U
 // (U = D L^T, plus the eliminated right-hand sides) for the back substitution, which
@@ -76,24 +77,32 @@ __device__ __forceinline__ float dpp(float v)
     return __int_as_float(dpp_i<CTRL>(__float_as_int(v)));
 }
 
-template <typename T>
-__device__ __forceinline__ void argmax_step(T& v, int& i, T ov, int oi)
+// argmax key of a non-negative magnitude: the value's leading bits (sign dropped, 8 or
+// 11 exponent bits, then mantissa bits) with the low 6 bits replaced by 63 - lane,
+// so an unsigned max picks the largest value and, among values equal to ~2^-14
+// relative, the lowest lane. 0 marks "no candidate". The exact magnitude is read
+// back from the chosen lane; only the choice among near-ties is coarsened.
+__device__ __forceinline__ uint32_t argmax_key(double v, int li)
 {
-    if (ov > v || (ov == v && oi < i)) { v = ov; i = oi; }
+    const uint32_t hi = (uint32_t)((unsigned long long)__double_as_longlong(v) >> 32);
+    return (hi & ~63u) | (uint32_t)(63 - li);
+}
+__device__ __forceinline__ uint32_t argmax_key(float v, int li)
+{
+    return ((uint32_t)__float_as_int(v) & ~63u) | (uint32_t)(63 - li);
 }
 
-// all-reduce (max value, lowest index on ties) over aligned groups of NMAX lanes;
-// every lane of the wave must be active
-template <int NMAX, typename T>
-__device__ __forceinline__ void group_argmax(T& v, int& i)
+// all-reduce of unsigned keys (max) over aligned groups of NMAX lanes
+template <int NMAX>
+__device__ __forceinline__ uint32_t group_max_u32(uint32_t k)
 {
-    // row_ror:1,2,4,8 inside each 16-lane row (DPP, no LDS round trip)
-    argmax_step(v, i, dpp<0x121>(v), dpp_i<0x121>(i));
-    argmax_step(v, i, dpp<0x122>(v), dpp_i<0x122>(i));
-    argmax_step(v, i, dpp<0x124>(v), dpp_i<0x124>(i));
-    argmax_step(v, i, dpp<0x128>(v), dpp_i<0x128>(i));
+    k = max(k, (uint32_t)dpp_i<0x121>((int)k));
+    k = max(k, (uint32_t)dpp_i<0x122>((int)k));
+    k = max(k, (uint32_t)dpp_i<0x124>((int)k));
+    k = max(k, (uint32_t)dpp_i<0x128>((int)k));
 #pragma unroll
-    for (int m = 16; m < NMAX; m <<= 1) argmax_step(v, i, (T)__shfl_xor(v, m, 64), __shfl_xor(i, m, 64));
+    for (int m = 16; m < NMAX; m <<= 1) k = max(k, (uint32_t)__shfl_xor((int)k, m, 64));
+    return k;
 }
 
 template <int NMAX>
@@ -578,12 +587,10 @@ k_build(const BuildDev P)
         const bool busy = gact != 0ull;             // this lane's group still factorises
         const int c0 = busy ? (int)__builtin_ctzll(gact) : 0;
         // column c0 = row c0 by symmetry: each lane reads its own entry
-        T v = (T)fabs(group_reg_get<NMAX>(r, c0));
-        if (!(act && li != c0)) v = (T)-1;
-        int vi = li;
-        group_argmax<NMAX>(v, vi);
-        T colmax = v > (T)0 ? v : (T)0;
-        int imax = vi;
+        const T cv = (T)fabs(group_reg_get<NMAX>(r, c0));
+        const uint32_t ck = group_max_u32<NMAX>((act && li != c0) ? argmax_key(cv, li) : 0u);
+        int imax = ck ? 63 - (int)(ck & 63u) : c0;
+        T colmax = ck ? group_bcast<NMAX>(cv, imax) : (T)0;
         const T absakk = (T)fabs(group_bcast<NMAX>(diag, c0));
         int kstep = 1, p = c0, kp = c0;
         bool need_rook = busy && absakk < alpha * colmax;
@@ -594,11 +601,10 @@ k_build(const BuildDev P)
         // rook iterations: every group takes part in the collective steps
         for (int it = 0; it < 2 * NMAX; ++it) {
             if (__ballot(need_rook) == 0ull) break;
-            T rv = (T)fabs(group_reg_get<NMAX>(r, imax));
-            if (!(need_rook && act && li != imax)) rv = (T)-1;
-            int ri = li;
-            group_argmax<NMAX>(rv, ri);
-            const T rowmax = rv > (T)0 ? rv : (T)0;
+            const T rv = (T)fabs(group_reg_get<NMAX>(r, imax));
+            const uint32_t rk = group_max_u32<NMAX>((need_rook && act && li != imax) ? argmax_key(rv, li) : 0u);
+            const int ri = rk ? 63 - (int)(rk & 63u) : imax;
+            const T rowmax = rk ? group_bcast<NMAX>(rv, ri) : (T)0;
             const T aii = (T)fabs(group_bcast<NMAX>(diag, imax));
             if (need_rook) {
                 if (!(aii < alpha * rowmax)) {

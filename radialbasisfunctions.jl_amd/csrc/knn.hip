@@ -110,13 +110,16 @@ __device__ __forceinline__ bool lt(double a, int ia, double b, int ib) { return 
 template <int DIM>
 __device__ __forceinline__ double sqdist(const double* a, const double (&q)[DIM])
 {
-    // no contraction: same rounding as sum((x .- y).^2) evaluated term by term
+    // no contraction: same rounding as sum((x .- y).^2) evaluated term by term. The
+    // operators must be written here, inside the pragma's scope: arithmetic helpers
+    // defined elsewhere keep their own contraction setting and were fused into FMAs.
 #pragma clang fp contract(off)
     double s = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
-        const double t = __dsub_rn(a[d], q[d]);
-        s = __dadd_rn(s, __dmul_rn(t, t));
+        const double t = a[d] - q[d];
+        const double t2 = t * t;
+        s = s + t2;
     }
     return s;
 }

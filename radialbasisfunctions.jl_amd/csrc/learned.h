@@ -55,9 +55,11 @@ struct LearnedDev {
 
 hipError_t launch_learned_build(const LearnedDev& P, hipStream_t s);
 
-// canonical offsets (n_eval * k * 2 FP64), network inputs (n_eval * kInPad BF16) and h
+// canonical offsets (n_eval * k * 2 FP64), network inputs (n_eval * kInPad BF16: the
+// 2k offsets' BF16 high parts, then their BF16 remainders, then zeros), the stencil
+// radius h and the canonical reference distance r1
 hipError_t launch_stencil_features(const double* data, const double* eval, int64_t n_eval, int k,
                                    const int32_t* adj_idx, double* out_xi, __bf16* out_x, double* out_h,
-                                   hipStream_t s);
+                                   double* out_r1, hipStream_t s);
 
 }  // namespace rbfx

@@ -2,10 +2,12 @@
 //
 // No reference counterpart. Each 256-thread workgroup takes 64 stencils:
 //   A. gather the k neighbours, scale by the stencil radius h, rotate/reflect into the
-//      canonical frame (canonical_rotation), write BF16 network inputs to LDS;
-//   B. evaluate the MLP (64 -> 256 -> 256 -> 256 -> 32, GELU) on the matrix cores
+//      canonical frame (canonical_rotation), write the network inputs (each offset as
+//      a BF16 high part plus a BF16 remainder) to LDS;
+//   B. evaluate the MLP (128 -> 256 -> 256 -> 256 -> 32, GELU) on the matrix cores
 //      (v_mfma_f32_16x16x32_bf16, activations in LDS, weights streamed from L2);
-//   C. in FP64: de-standardise the outputs to (w~, lambda), project w~ onto exact
+//   C. in FP64: de-standardise the outputs (the network predicts w~ r1^2, r1 the
+//      distance of the canonical reference neighbour) to w~, project w~ onto exact
 //      polynomial reproduction (P^T w~ = L P, minimal-norm correction through the
 //      Gram matrix G = P^T P), refit lambda by least squares, and form the collocation
 //      residual s - P lambda with s = L phi - Phi w~ (O(k^2) against the O(k^3) solve);
@@ -151,11 +153,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         }
     }
     __syncthreads();
-    double c = 1.0, s = 0.0;
+    double c = 1.0, s = 0.0, r1 = 1.0;
     for (int i = 0; i < K; ++i) {
         const double x = s_xi[sl][i][0], y = s_xi[sl][i][1];
         if (x != 0.0 || y != 0.0) {
             canonical_rotation(x, y, c, s);
+            r1 = sqrt(x * x + y * y);
             break;
         }
     }
@@ -178,11 +181,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         if (i < K) {
             s_xi[sl][i][0] = dx[r];
             s_xi[sl][i][1] = dy[r];
-            s_act[sl][2 * i] = (__bf16)(float)dx[r];
-            s_act[sl][2 * i + 1] = (__bf16)(float)dy[r];
+            // BF16 high part, then the BF16 rounding of the remainder
+            const __bf16 hx = (__bf16)(float)dx[r], hy = (__bf16)(float)dy[r];
+            s_act[sl][2 * i] = hx;
+            s_act[sl][2 * i + 1] = hy;
+            s_act[sl][2 * K + 2 * i] = (__bf16)(float)(dx[r] - (double)(float)hx);
+            s_act[sl][2 * K + 2 * i + 1] = (__bf16)(float)(dy[r] - (double)(float)hy);
         }
     }
-    for (int col = 2 * K + q; col < kInPad; col += 4) s_act[sl][col] = (__bf16)0.0f;
+    for (int col = 4 * K + q; col < kInPad; col += 4) s_act[sl][col] = (__bf16)0.0f;
     __syncthreads();
 
     // ---- B. network
@@ -218,7 +225,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         const int i = q + 4 * r;
         wt[r] = 0.0;
         if (i < K) {
-            wt[r] = (double)s_out[sl][i] * P.out_std[i] + P.out_mean[i];
+            // the network predicts w~ r1^2 (r1: the canonical reference distance)
+            wt[r] = ((double)s_out[sl][i] * P.out_std[i] + P.out_mean[i]) / (r1 * r1);
             double pr[MM];
 #pragma unroll
             for (int a = 0; a < MM; ++a) pr[a] = mono2(s_mono[a], dx[r], dy[r]);
@@ -374,7 +382,8 @@ hipError_t launch_learned_build(const LearnedDev& P, hipStream_t s)
 // ---- canonical features for training data: one thread per stencil, the same
 // arithmetic as phase A above (quad partial sums of the y offsets included)
 __global__ __launch_bounds__(256) void k_features(const double* data, const double* eval, int64_t n_eval, int k,
-                                                  const int32_t* adj, double* out_xi, __bf16* out_x, double* out_h)
+                                                  const int32_t* adj, double* out_xi, __bf16* out_x, double* out_h,
+                                                  double* out_r1)
 {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
     if (e >= n_eval) return;
@@ -387,12 +396,13 @@ __global__ __launch_bounds__(256) void k_features(const double* data, const doub
     }
     double h = sqrt(fmax(fmax(hq[0], hq[1]), fmax(hq[2], hq[3])));
     if (!(h > 0.0)) h = 1.0;
-    double c = 1.0, s = 0.0;
+    double c = 1.0, s = 0.0, r1 = 1.0;
     for (int i = 0; i < k; ++i) {
         const int64_t j = adj[e * k + i];
         const double x = (data[2 * j] - xe) / h, y = (data[2 * j + 1] - ye) / h;
         if (x != 0.0 || y != 0.0) {
             canonical_rotation(x, y, c, s);
+            r1 = sqrt(x * x + y * y);
             break;
         }
     }
@@ -411,22 +421,27 @@ __global__ __launch_bounds__(256) void k_features(const double* data, const doub
             out_xi[(e * k + i) * 2] = xr;
             out_xi[(e * k + i) * 2 + 1] = yr;
         }
-        if (out_x && 2 * i + 1 < kInPad) {
-            out_x[e * kInPad + 2 * i] = (__bf16)(float)xr;
-            out_x[e * kInPad + 2 * i + 1] = (__bf16)(float)yr;
+        if (out_x && 4 * k <= kInPad) {
+            const __bf16 hx = (__bf16)(float)xr, hy = (__bf16)(float)yr;
+            out_x[e * kInPad + 2 * i] = hx;
+            out_x[e * kInPad + 2 * i + 1] = hy;
+            out_x[e * kInPad + 2 * k + 2 * i] = (__bf16)(float)(xr - (double)(float)hx);
+            out_x[e * kInPad + 2 * k + 2 * i + 1] = (__bf16)(float)(yr - (double)(float)hy);
         }
     }
     if (out_x)
-        for (int col = 2 * k; col < kInPad; ++col) out_x[e * kInPad + col] = (__bf16)0.0f;
+        for (int col = 4 * k; col < kInPad; ++col) out_x[e * kInPad + col] = (__bf16)0.0f;
     if (out_h) out_h[e] = h;
+    if (out_r1) out_r1[e] = r1;
 }
 
 hipError_t launch_stencil_features(const double* data, const double* eval, int64_t n_eval, int k,
-                                   const int32_t* adj_idx, double* out_xi, __bf16* out_x, double* out_h, hipStream_t s)
+                                   const int32_t* adj_idx, double* out_xi, __bf16* out_x, double* out_h,
+                                   double* out_r1, hipStream_t s)
 {
     if (n_eval <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_features, dim3((unsigned)((n_eval + 255) / 256)), dim3(256), 0, s, data, eval, n_eval, k,
-                       adj_idx, out_xi, out_x, out_h);
+                       adj_idx, out_xi, out_x, out_h, out_r1);
     return hipGetLastError();
 }
 

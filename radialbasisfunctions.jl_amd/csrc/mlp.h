@@ -1,8 +1,10 @@
 // Internal interface of the stencil-weight MLP (training and inference kernels).
 //
-// Network: in (2k offsets, zero-padded to kInPad) -> kHid -> kHid -> kHid -> out
-// (k weights + m multipliers, zero-padded to kOutPad); GELU (tanh form) on the
-// hidden layers; BF16 operands, FP32 accumulation on the matrix cores.
+// Network: in (2k offsets, each split into a BF16 high part and a BF16 remainder, so
+// 4k inputs zero-padded to kInPad) -> kHid -> kHid -> kHid -> out (k weights + m
+// multipliers, zero-padded to kOutPad); GELU (tanh form) on the hidden layers; BF16
+// operands, FP32 accumulation on the matrix cores. The split keeps ~16 mantissa bits
+// of the offsets, which the 1/r^2 sensitivity of the weights needs.
 //
 // Parameters (FP32 master copy, one flat array): for each layer l, Wt_l [N_l x K_l]
 // (output-major, the reduction index contiguous) followed by b_l [N_l]. The BF16
@@ -14,7 +16,7 @@
 namespace rbfx {
 
 constexpr int kMlpLayers = 4;
-constexpr int kInPad = 64;
+constexpr int kInPad = 128;
 constexpr int kHid = 256;
 constexpr int kOutPad = 32;
 
@@ -68,7 +70,7 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s);
 
 // training step pieces
 hipError_t launch_mse_grad(int M, int n_out, const float* Y, int64_t ldy, const float* T, int64_t ldt,
-                           __bf16* dY, int64_t ld_dy, float* loss, hipStream_t s);
+                           const float* row_weight, __bf16* dY, int64_t ld_dy, float* loss, hipStream_t s);
 hipError_t launch_colsum_bf16(int M, int N, const __bf16* X, int64_t ldx, float* out, hipStream_t s);
 hipError_t launch_adam(int64_t n, float* p, const float* g, float* m, float* v, float lr, float b1, float b2,
                        float eps, int t, hipStream_t s);

@@ -111,9 +111,11 @@ hipError_t launch_gemm(const GemmArgs& g, hipStream_t s)
     return hipGetLastError();
 }
 
-// dY = 2 (Y - T) / (M n_out) as BF16 (zero in the padding columns), loss += mean (Y-T)^2
+// loss = mean_m w_m mean_n (Y-T)^2 (w_m = 1 without row weights);
+// dY = 2 w_m (Y - T) / (M n_out) as BF16 (zero in the padding columns)
 __global__ __launch_bounds__(256) void k_mse_grad(int M, int n_out, const float* Y, int64_t ldy, const float* T,
-                                                  int64_t ldt, __bf16* dY, int64_t ld_dy, float* loss)
+                                                  int64_t ldt, const float* row_weight, __bf16* dY, int64_t ld_dy,
+                                                  float* loss)
 {
     __shared__ float red[256];
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -122,12 +124,13 @@ __global__ __launch_bounds__(256) void k_mse_grad(int M, int n_out, const float*
     if (i < total) {
         const int64_t m = i / ld_dy;
         const int n = (int)(i % ld_dy);
+        const float wm = row_weight ? row_weight[m] : 1.0f;
         float d = 0.f;
         if (n < n_out) {
             d = Y[m * ldy + n] - T[m * ldt + n];
-            l = d * d;
+            l = wm * d * d;
         }
-        dY[i] = (__bf16)(2.0f * d / ((float)M * (float)n_out));
+        dY[i] = (__bf16)(2.0f * wm * d / ((float)M * (float)n_out));
     }
     red[threadIdx.x] = l;
     __syncthreads();
@@ -138,12 +141,12 @@ __global__ __launch_bounds__(256) void k_mse_grad(int M, int n_out, const float*
     if (threadIdx.x == 0) atomicAdd(loss, red[0] / ((float)M * (float)n_out));
 }
 
-hipError_t launch_mse_grad(int M, int n_out, const float* Y, int64_t ldy, const float* T, int64_t ldt, __bf16* dY,
-                           int64_t ld_dy, float* loss, hipStream_t s)
+hipError_t launch_mse_grad(int M, int n_out, const float* Y, int64_t ldy, const float* T, int64_t ldt,
+                           const float* row_weight, __bf16* dY, int64_t ld_dy, float* loss, hipStream_t s)
 {
     const int64_t total = (int64_t)M * ld_dy;
     hipLaunchKernelGGL(k_mse_grad, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, M, n_out, Y, ldy, T, ldt,
-                       dY, ld_dy, loss);
+                       row_weight, dY, ld_dy, loss);
     return hipGetLastError();
 }
 

@@ -20,8 +20,15 @@ struct SpmvDev {
     int64_t y_ld_point, y_ld_field, y_ld_val;
     double alpha, beta;
     const int32_t* row_order;  // optional: processing order of the rows (spatial order)
+    int rows_permuted;         // col_idx / vals / row_ptr are stored in processing order
+    int64_t max_row;           // longest row (0: unknown)
+    int mapping;               // 0 automatic, 1 lanes per row reading HBM, 2 staged through LDS
 };
 
 hipError_t launch_spmv(const SpmvDev& P, hipStream_t s);
+
+// copy fixed-length CSR rows into processing order: out row r = in row row_order[r]
+hipError_t launch_permute_rows(int64_t n_rows, int k, const int32_t* row_order, const int32_t* col, const double* vals,
+                               int64_t nnz, int n_valarrays, int32_t* out_col, double* out_vals, hipStream_t s);
 
 }  // namespace rbfx

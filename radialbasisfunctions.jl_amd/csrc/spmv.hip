@@ -44,8 +44,9 @@ __global__ __launch_bounds__(256) void k_spmv_fields(const SpmvDev P, int G /* p
     const int64_t r = lb * rows_per_block + lane / G;
     if (r >= P.n_rows || f >= P.n_fields) return;
     const int64_t row = P.row_order ? (int64_t)P.row_order[r] : r;
-    const int64_t b = P.row_ptr ? P.row_ptr[row] : row * (int64_t)P.k;
-    const int64_t e = P.row_ptr ? P.row_ptr[row + 1] : b + P.k;
+    const int64_t srow = P.rows_permuted ? r : row;  // where the row's entries are stored
+    const int64_t b = P.row_ptr ? P.row_ptr[srow] : srow * (int64_t)P.k;
+    const int64_t e = P.row_ptr ? P.row_ptr[srow + 1] : b + P.k;
     double acc[NV];
 #pragma unroll
     for (int v = 0; v < NV; ++v) acc[v] = 0.0;
@@ -73,14 +74,15 @@ __global__ __launch_bounds__(256) void k_spmv_rows(const SpmvDev P, int64_t nb_r
     const int64_t r = (lb * blockDim.x + threadIdx.x) / G;
     const bool live = r < P.n_rows;
     const int64_t row = live ? (P.row_order ? (int64_t)P.row_order[r] : r) : 0;
+    const int64_t srow = P.rows_permuted ? r : row;  // where the row's entries are stored
     double acc[NV][FMAX];
 #pragma unroll
     for (int v = 0; v < NV; ++v)
 #pragma unroll
         for (int f = 0; f < FMAX; ++f) acc[v][f] = 0.0;
     if (live) {
-        const int64_t b = P.row_ptr ? P.row_ptr[row] : row * (int64_t)P.k;
-        const int64_t e = P.row_ptr ? P.row_ptr[row + 1] : b + P.k;
+        const int64_t b = P.row_ptr ? P.row_ptr[srow] : srow * (int64_t)P.k;
+        const int64_t e = P.row_ptr ? P.row_ptr[srow + 1] : b + P.k;
         for (int64_t q = b + sub; q < e; q += G) {
             const int64_t c = __builtin_nontemporal_load(P.col_idx + q);
             double xv[FMAX];
@@ -114,10 +116,145 @@ __global__ __launch_bounds__(256) void k_spmv_rows(const SpmvDev P, int64_t nb_r
     }
 }
 
+// Staged mapping, for rows stored back to back in processing order: a workgroup owns
+// R = 256/G consecutive rows, whose entries [b0, b1) it first copies into LDS with
+// consecutive lanes on consecutive entries, so every load instruction of the index and
+// value streams reads whole cache lines exactly once; then G lanes per row gather X
+// (U independent gathers in flight per lane) and reduce as in k_spmv_rows.
+// G lanes per row: gather X (U independent gathers in flight per lane), accumulate
+// the row's products, butterfly-reduce inside the group and write Y. Entries are read
+// from `col` / `val` (value array v at val + v * vstride) at offsets relative to b0.
+template <int NV, int G, int FMAX, int U, typename ColPtr, typename ValPtr>
+__device__ __forceinline__ void rows_gather_reduce(const SpmvDev& P, int64_t r0, int64_t r1, int64_t b0, ColPtr col,
+                                                   ValPtr val, int64_t vstride)
+{
+    const int sub = threadIdx.x % G;
+    const int64_t r = r0 + threadIdx.x / G;
+    const bool live = r < r1;
+    const int64_t row = live ? (P.row_order ? (int64_t)P.row_order[r] : r) : 0;
+    double acc[NV][FMAX];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int f = 0; f < FMAX; ++f) acc[v][f] = 0.0;
+    if (live) {
+        const int64_t qb = (P.row_ptr ? P.row_ptr[r] : r * (int64_t)P.k) - b0;
+        const int64_t qe = (P.row_ptr ? P.row_ptr[r + 1] : (r + 1) * (int64_t)P.k) - b0;
+        for (int64_t q0 = qb + sub; q0 < qe; q0 += U * G) {
+            double xv[U][FMAX], w[U][NV];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t q = q0 + u * G;
+                const bool ok = q < qe;
+                const int64_t c = ok ? (int64_t)col[q] : 0;
+#pragma unroll
+                for (int f = 0; f < FMAX; ++f)
+                    xv[u][f] = (ok && f < P.n_fields) ? P.x[c * P.x_ld_point + (int64_t)f * P.x_ld_field] : 0.0;
+#pragma unroll
+                for (int v = 0; v < NV; ++v) w[u][v] = ok ? val[v * vstride + q] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int f = 0; f < FMAX; ++f) acc[v][f] += w[u][v] * xv[u][f];
+        }
+    }
+#pragma unroll
+    for (int m = G / 2; m >= 1; m >>= 1)
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int f = 0; f < FMAX; ++f) acc[v][f] += __shfl_xor(acc[v][f], m, G);
+    if (live && sub == 0) {
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int f = 0; f < FMAX; ++f) {
+                if (f < P.n_fields) {
+                    double* yp = P.y + (int64_t)v * P.y_ld_val + row * P.y_ld_point + (int64_t)f * P.y_ld_field;
+                    const double out = P.alpha * acc[v][f];
+                    *yp = (P.beta == 0.0) ? out : out + P.beta * *yp;
+                }
+            }
+    }
+}
+
+// Staged mapping, for rows stored back to back in processing order: a workgroup owns
+// R = 256/G consecutive rows, whose entries [b0, b1) it first copies into LDS with
+// consecutive lanes on consecutive entries, so every load instruction of the index and
+// value streams reads whole cache lines exactly once; then G lanes per row gather X.
+// A workgroup whose entries exceed the LDS capacity (a longest-row bound that was too
+// small) reads them from HBM instead.
+template <int NV, int G, int FMAX, int U>
+__global__ __launch_bounds__(256) void k_spmv_staged(const SpmvDev P, int64_t nb_real, int cap)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+    double* s_val = reinterpret_cast<double*>(s_raw);                        // NV x cap
+    int32_t* s_col = reinterpret_cast<int32_t*>(s_val + (int64_t)NV * cap);  // cap
+    constexpr int R = 256 / G;
+    const int64_t lb = xcd_block(nb_real);
+    if (lb < 0) return;  // whole block
+    const int64_t r0 = lb * R;
+    const int64_t r1 = r0 + R < P.n_rows ? r0 + R : P.n_rows;
+    const int64_t b0 = P.row_ptr ? P.row_ptr[r0] : r0 * (int64_t)P.k;
+    const int64_t b1 = P.row_ptr ? P.row_ptr[r1] : r1 * (int64_t)P.k;
+    const int64_t n_e = b1 - b0;
+    if (n_e > cap) {
+        rows_gather_reduce<NV, G, FMAX, U>(P, r0, r1, 0, P.col_idx, P.vals, P.nnz);
+        return;
+    }
+    for (int i = threadIdx.x; i < n_e; i += 256) {
+        s_col[i] = __builtin_nontemporal_load(P.col_idx + b0 + i);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) s_val[v * cap + i] = __builtin_nontemporal_load(P.vals + (int64_t)v * P.nnz + b0 + i);
+    }
+    __syncthreads();
+    rows_gather_reduce<NV, G, FMAX, U>(P, r0, r1, b0, (const int32_t*)s_col, (const double*)s_val, (int64_t)cap);
+}
+
+// LDS bytes of one staged workgroup
+static int64_t staged_lds(int nv, int G, int64_t max_row) { return (256 / G) * max_row * (8 * (int64_t)nv + 4); }
+constexpr int64_t kStagedLdsMax = 48 * 1024;
+
+template <int NV, int FMAX>
+static hipError_t spmv_staged(const SpmvDev& P, hipStream_t s)
+{
+    const int64_t L = P.max_row;
+#define RBFX_STAGED(GG, UU)                                                                                      \
+    {                                                                                                            \
+        const int64_t nbr = (P.n_rows + 256 / (GG) - 1) / (256 / (GG));                                          \
+        const int cap = (int)((256 / (GG)) * L);                                                                 \
+        hipLaunchKernelGGL((k_spmv_staged<NV, GG, FMAX, UU>), dim3((unsigned)pad8(nbr)), dim3(256),              \
+                           (unsigned)staged_lds(NV, GG, L), s, P, nbr, cap);                                     \
+        return hipGetLastError();                                                                                \
+    }
+    if (L <= 16) RBFX_STAGED(4, 4)
+    if (L <= 20) RBFX_STAGED(4, 5)
+    if (L <= 32) RBFX_STAGED(8, 4)
+    if (L <= 40) RBFX_STAGED(8, 5)
+    RBFX_STAGED(16, 4)
+#undef RBFX_STAGED
+}
+
+// lanes per row of the staged mapping for the longest row L (as chosen in spmv_staged)
+static int staged_g(int64_t L) { return L <= 20 ? 4 : (L <= 40 ? 8 : 16); }
+
 template <int NV>
 static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
 {
     const int TB = 256;
+    // staged mapping: rows stored in processing order, a known longest row, LDS in budget
+    const bool contiguous = !P.row_order || P.rows_permuted;
+    const bool staged_ok = contiguous && P.n_fields < 8 && P.max_row > 0 && P.nnz > 0 &&
+                           staged_lds(NV, staged_g(P.max_row), P.max_row) <= kStagedLdsMax;
+    if (P.mapping == 2 && !staged_ok) return hipErrorInvalidValue;
+    if (staged_ok && P.mapping != 1) {
+        if (P.n_fields <= 1) return spmv_staged<NV, 1>(P, s);
+        if (P.n_fields <= 4) return spmv_staged<NV, 4>(P, s);
+        return spmv_staged<NV, 8>(P, s);
+    }
     if (P.n_fields >= 8) {
         int G = 8;
         while (G < P.n_fields) G <<= 1;
@@ -147,6 +284,30 @@ static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
     if (avg_row < 40) { RBFX_SPMV_LAUNCH(8, 8, threads_g8) }
     RBFX_SPMV_LAUNCH(16, 8, threads_g16)
 #undef RBFX_SPMV_LAUNCH
+}
+
+// one thread per (processing row, entry): streaming reads of the source row, writes in order
+__global__ __launch_bounds__(256) void k_permute_rows(int64_t n_rows, int k, const int32_t* __restrict__ order,
+                                                      const int32_t* __restrict__ col, const double* __restrict__ vals,
+                                                      int64_t nnz, int nv, int32_t* __restrict__ out_col,
+                                                      double* __restrict__ out_vals)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_rows * k) return;
+    const int64_t r = i / k, j = i % k;
+    const int64_t src = (int64_t)order[r] * k + j;
+    out_col[i] = col[src];
+    for (int v = 0; v < nv; ++v) out_vals[(int64_t)v * nnz + i] = vals[(int64_t)v * nnz + src];
+}
+
+hipError_t launch_permute_rows(int64_t n_rows, int k, const int32_t* row_order, const int32_t* col, const double* vals,
+                               int64_t nnz, int n_valarrays, int32_t* out_col, double* out_vals, hipStream_t s)
+{
+    const int64_t n = n_rows * k;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_permute_rows, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n_rows, k, row_order, col,
+                       vals, nnz, n_valarrays, out_col, out_vals);
+    return hipGetLastError();
 }
 
 hipError_t launch_spmv(const SpmvDev& P, hipStream_t s)

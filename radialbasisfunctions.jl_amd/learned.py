@@ -1,13 +1,16 @@
 """Learned stencil-weight surrogate: model container, training data and GPU training.
 
-No counterpart in the reference package. A BF16 MLP (2k=40 canonical offsets ->
-256 -> 256 -> 256 -> k+m=26, GELU) maps the canonical, radius-scaled stencil offsets to
-the solution (w~, lambda) of the scaled saddle-point system. Training targets come from
-the plain FP64 solver run on the canonical stencils themselves; forward and backward
-passes and the Adam update run in the library's own kernels (rbfx_mlp_run,
-rbfx_mlp_adam). At build time (precision="learned") each prediction is projected onto
-exact polynomial reproduction in FP64, checked by its collocation residual, and
-stencils above `accept_tol` are re-solved by the FP32-refined solver.
+No counterpart in the reference package. A BF16 MLP maps the canonical, radius-scaled
+stencil offsets (2k = 40 values, each given as a BF16 high part and a BF16 remainder)
+through 256 -> 256 -> 256 (GELU) to the solution (w~, lambda) of the scaled
+saddle-point system, multiplied by r1^2 (r1: the distance of the canonical reference
+neighbour, which carries most of the weights' 1/r^2 scale). Training targets come from
+the plain FP64 solver run on the canonical stencils themselves; the loss is the mean
+squared error relative to each stencil's weight magnitude. Forward and backward passes
+and the Adam update run in the library's own kernels (rbfx_mlp_run, rbfx_mlp_adam).
+At build time (precision="learned") each prediction is projected onto exact
+polynomial reproduction in FP64, checked by its collocation residual, and stencils
+above `accept_tol` are re-solved by the FP32-refined solver.
 """
 import ctypes as C
 import math
@@ -63,7 +66,7 @@ class LearnedModel:
         n_out = k + N_MONO_DEG2_2D
         for l in range(abi.MLP_LAYERS):
             K, N = abi.mlp_k(l), abi.mlp_n(l)
-            fan_in = 2 * k if l == 0 else K
+            fan_in = 4 * k if l == 0 else K
             fan_out = n_out if l == abi.MLP_LAYERS - 1 else N
             a = math.sqrt(6.0 / (fan_in + fan_out))
             W = np.zeros((N, K), dtype=np.float32)
@@ -146,9 +149,10 @@ def canonical_targets(xi, device):
 
 
 def training_set(n_stencils, seed=1234, k=DEFAULT_K, device=None, clouds=4):
-    """Inputs (BF16 (n,64)) and targets (float64 (n, k+6)) from `clouds` seeded density
-    clouds (x10 density variation along x, y, and reversed), each stencil in its
-    canonical frame."""
+    """Inputs (BF16 (n,128)), targets (float64 (n, k+6): (w~, lambda) r1^2) and
+    per-stencil loss weights (float32 (n,): 1 / mean(w~^2 r1^4), mean 1) from `clouds`
+    seeded density clouds (x10 density variation along x, y, and reversed), each
+    stencil in its canonical frame."""
     t = _lib.torch()
     dev = _lib.device_for(device)
     per = int(math.ceil(n_stencils / clouds))
@@ -157,21 +161,24 @@ def training_set(n_stencils, seed=1234, k=DEFAULT_K, device=None, clouds=4):
         pts = density_cloud(per, seed + c, axis=c % 2, flip=(c // 2) % 2 == 1)
         d = t.as_tensor(pts, device=dev)
         adj = _lib.knn(d, d, k)
-        xi, x, _ = _lib.stencil_features(d, d, adj, k)
+        xi, x, _, r1 = _lib.stencil_features(d, d, adj, k)
         w, lam = canonical_targets(xi, dev)
         Xs.append(x)
-        Ys.append(t.cat([w, lam], 1))
+        Ys.append(t.cat([w, lam], 1) * (r1 * r1)[:, None])
         del d, adj, xi
     X = t.cat(Xs)[:n_stencils].contiguous()
     Y = t.cat(Ys)[:n_stencils].contiguous()
-    return X, Y
+    rw = 1.0 / (Y[:, :k] ** 2).mean(1).clamp_min(1e-300)
+    rw = (rw / rw.mean()).float().contiguous()
+    return X, Y, rw
 
 
 # ---- training ------------------------------------------------------------------
 def train(X, Y, steps=12000, batch=4096, lr=1e-3, lr_min=2e-5, seed=0, k=DEFAULT_K, device=None, log_every=1000,
-          model=None, log=print):
-    """Adam on the standardised targets with a cosine learning-rate schedule; every
-    pass runs in the library's MFMA kernels. Returns (model, loss history)."""
+          model=None, log=print, row_weight=None):
+    """Adam on the standardised targets (row-weighted squared error) with a cosine
+    learning-rate schedule; every pass runs in the library's MFMA kernels.
+    Returns (model, loss history)."""
     t = _lib.torch()
     dev = _lib.device_for(device)
     mean = Y.mean(0)
@@ -200,10 +207,12 @@ def train(X, Y, steps=12000, batch=4096, lr=1e-3, lr_min=2e-5, seed=0, k=DEFAULT
         pos += batch
         xb = X[idx].contiguous()
         zb = Z[idx].contiguous()
-        _lib.mlp_run(xb, model.params, model.wbf16, ws, batch, target=zb, grads=grads, loss=loss, n_out=n_out)
+        wb = None if row_weight is None else row_weight[idx].contiguous()
+        _lib.mlp_run(xb, model.params, model.wbf16, ws, batch, target=zb, grads=grads, loss=loss, n_out=n_out,
+                     row_weight=wb)
         cur = lr_min + 0.5 * (lr - lr_min) * (1.0 + math.cos(math.pi * (step - 1) / steps))
         _lib.mlp_adam(model.params, grads, m, v, model.wbf16, cur, 0.9, 0.999, 1e-8, step)
-        if step % log_every == 0 or step == steps:
+        if step == 1 or step % log_every == 0 or step == steps:
             lv = float(loss.item())
             hist.append((step, lv))
             if log:
@@ -212,7 +221,8 @@ def train(X, Y, steps=12000, batch=4096, lr=1e-3, lr_min=2e-5, seed=0, k=DEFAULT
 
 
 def weight_errors(model, X, Y):
-    """Per-stencil relative max-norm error of the predicted w~ against the solver's."""
+    """Per-stencil relative max-norm error of the predicted weights against the
+    solver's (both in the r1^2-scaled units of the targets; the ratio is scale-free)."""
     k = model.k
     P = model.predict(X)
     w_true = Y[:, :k]
@@ -266,8 +276,8 @@ def train_default(device=None, n_train=2_000_000, steps=20000, batch=4096, seed=
     """The 2-D Laplacian surrogate (PHS3, degree 2, k=20): 2M training stencils from
     seeded density clouds, Adam with a cosine schedule, then tolerance calibration."""
     dev = _lib.device_for(device)
-    X, Y = training_set(n_train, seed=seed, device=dev)
-    model, hist = train(X, Y, steps=steps, batch=batch, seed=seed, device=dev, log=log)
-    del X, Y
+    X, Y, rw = training_set(n_train, seed=seed, device=dev)
+    model, hist = train(X, Y, steps=steps, batch=batch, seed=seed, device=dev, log=log, row_weight=rw)
+    del X, Y, rw
     sweep = calibrate(model, device=dev, log=log)
     return model, hist, sweep

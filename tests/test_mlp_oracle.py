@@ -49,14 +49,15 @@ def test_backward_matches_finite_differences():
     p = small_params(1)
     M, n_out = 6, 26
     X = np.zeros((M, MO.IN_PAD))
-    X[:, :40] = rng.uniform(-1, 1, (M, 40))
+    X[:, :80] = rng.uniform(-1, 1, (M, 80))
     T = rng.standard_normal((M, MO.OUT_PAD))
-    loss, g = MO.loss_and_grad(p, X, T, n_out, bf16_round=False)
+    rw = rng.uniform(0.5, 2.0, M)
+    loss, g = MO.loss_and_grad(p, X, T, n_out, bf16_round=False, row_weight=rw)
     # check a spread of coordinates in every layer (weights and biases)
     idx = []
     for l in range(MO.LAYERS):
         n, k = MO.n_of(l), MO.k_of(l)
-        fan_in = 40 if l == 0 else k
+        fan_in = 80 if l == 0 else k
         rows = n_out if l == MO.LAYERS - 1 else n
         for _ in range(6):
             idx.append(MO.w_off(l) + rng.integers(rows) * k + rng.integers(fan_in))
@@ -66,11 +67,11 @@ def test_backward_matches_finite_differences():
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        fd = (MO.loss_and_grad(pp, X, T, n_out, False)[0] - MO.loss_and_grad(pm, X, T, n_out, False)[0]) / (2 * h)
+        fd = (MO.loss_and_grad(pp, X, T, n_out, False, rw)[0] - MO.loss_and_grad(pm, X, T, n_out, False, rw)[0]) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, g[i])
     # padded input columns and padded output rows carry no gradient
-    W0 = g[MO.w_off(0):MO.w_off(0) + 256 * 64].reshape(256, 64)
-    assert np.all(W0[:, 40:] == 0)
+    W0 = g[MO.w_off(0):MO.w_off(0) + 256 * 128].reshape(256, 128)
+    assert np.all(W0[:, 80:] == 0)
     W3 = g[MO.w_off(3):MO.w_off(3) + 32 * 256].reshape(32, 256)
     assert np.all(W3[n_out:] == 0)
 
@@ -95,13 +96,14 @@ def _stencil(seed, k=20):
 
 def test_canonical_offsets_invariant():
     pts, e, adj = _stencil(5)
-    xi, h = MO.canonical_offsets(pts, pts[[e]], adj[None])
+    xi, h, r1 = MO.canonical_offsets(pts, pts[[e]], adj[None])
     th = 0.7
     R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
     for T in (R, np.diag([1.0, -1.0]) @ R, np.diag([-1.0, 1.0])):
         q = 3.0 * (pts - pts[e]) @ T.T + np.array([5.0, -2.0])
-        xi2, h2 = MO.canonical_offsets(q, q[[e]], adj[None])
+        xi2, h2, r12 = MO.canonical_offsets(q, q[[e]], adj[None])
         assert np.allclose(xi2, xi, atol=1e-12)
+        assert np.isclose(r12[0], r1[0], rtol=1e-12)
         assert np.isclose(h2[0], 3.0 * h[0], rtol=1e-12)
     # the first nonzero offset lies on +x, and the y offsets sum to >= 0
     assert xi[0, 0, 0] == 0 and xi[0, 0, 1] == 0
