@@ -39,7 +39,110 @@ def mlp_flops(k=20, m=6, hid=256):
     return 2 * (2 * k * hid + 2 * hid * hid + hid * (k + m))
 
 
+TRAFFIC_FILE = ROOT / "profiles" / "laplacian2d_4M_hbm_traffic.txt"
+
+
+def _runs(vals, tol=0.1):
+    """Consecutive dispatches grouped while each stays within tol of its group's first
+    value: [(count, mean), ...] (launches of one kernel differ in size)."""
+    groups = []
+    for v in vals:
+        if groups and abs(v - groups[-1][1]) <= tol * max(abs(groups[-1][1]), 1e-30):
+            groups[-1][0] += 1
+            groups[-1][2] += v
+        else:
+            groups.append([1, v, v])
+    return [(c, tot / c) for c, _, tot in groups]
+
+
+def summarize(path):
+    """Per-kernel table of a rocprofv3 output directory (its *_results.db files): device
+    time per kernel for a --kernel-trace run, counter value per dispatch for --pmc;
+    dispatches in launch order, consecutive ones of similar size grouped as count x mean."""
+    import collections
+    import glob
+    import sqlite3
+
+    dbs = sorted(glob.glob(os.path.join(path, "**", "*_results.db"), recursive=True))
+    if not dbs:
+        sys.exit(f"no *_results.db under {path}")
+    for db in dbs:
+        cur = sqlite3.connect(db).cursor()
+        tables = {r[0] for r in cur.execute("select name from sqlite_master where type in ('table', 'view')")}
+        if "pmc_events" in tables and cur.execute("select count(*) from pmc_events").fetchone()[0]:
+            per = collections.OrderedDict()  # (dispatch, counter) -> [kernel, sum over instances]
+            for name, did, cn, v in cur.execute(
+                    "select name, dispatch_id, counter_name, counter_value from pmc_events order by dispatch_id"):
+                per.setdefault((did, cn), [name, 0.0])[1] += v
+            agg = collections.defaultdict(lambda: collections.defaultdict(list))
+            for (did, cn), (name, v) in per.items():
+                agg[name][cn].append(v)
+            print(f"{'counter':12s} {'mean/dispatch':>14s} {'dispatches':>10s}  kernel")
+            for name, d in agg.items():
+                for cn, vals in d.items():
+                    print(f"{cn:12s} {sum(vals) / len(vals):14.6g} {len(vals):10d}  {name[:110]}")
+                    print(" " * 14 + "runs (count x mean): " +
+                          ", ".join(f"{c} x {m:.6g}" for c, m in _runs(vals)))
+        else:
+            agg = collections.OrderedDict()
+            for name, dur in cur.execute("select name, duration from kernels order by start"):
+                agg.setdefault(name, []).append(dur)
+            print(f"{'total ms':>10s} {'calls':>6s} {'avg ms':>10s}  kernel")
+            for name, durs in sorted(agg.items(), key=lambda kv: -sum(kv[1])):
+                tot = sum(durs)
+                print(f"{tot / 1e6:10.3f} {len(durs):6d} {tot / len(durs) / 1e6:10.4f}  {name[:110]}")
+                if tot > 1e6:
+                    print(" " * 18 + "runs (count x mean ms): " +
+                          ", ".join(f"{c} x {m / 1e6:.4f}" for c, m in _runs(durs, 0.2)))
+
+
+def copy_calibration(gib=1, reps=5):
+    """A plain device copy of known size: what FETCH_SIZE / WRITE_SIZE read for it gives
+    the factor from counter KiB to bytes moved."""
+    import torch
+
+    n = (gib << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_each_way": n * 8, "reps": reps}))
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (a name substring) from the committed rocprofv3
+    summary: lines 'traffic <kernel> fetch_bytes=<B> write_bytes=<B>'."""
+    if not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        parts = line.split()
+        if len(parts) == 4 and parts[0] == "traffic" and parts[1] == kernel:
+            kv = dict(p.split("=") for p in parts[2:])
+            return float(kv["fetch_bytes"]) + float(kv["write_bytes"])
+    return None
+
+
+def time_apply(op, X, stream, reps=20):
+    import torch
+
+    for _ in range(3):
+        op._spmv(X)
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(reps):
+        op._spmv(X)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / reps
+
+
 def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--summarize":
+        return summarize(sys.argv[2])
+    if len(sys.argv) == 2 and sys.argv[1] == "--copy-calibration":
+        return copy_calibration()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
@@ -124,23 +227,23 @@ def main():
                                status={"ok": sc[0], "refined": sc[2], "fallback64": sc[3], "learned": sc[4]})
         ops[method] = op
 
-    # apply: y = W f (Laplacian, 1 field)
+    # apply: y = W f (Laplacian, 1 field), rows walked in spatial order (apply plan)
     op = ops["fp64"]
     X = d_f[:, None].contiguous()
-    for _ in range(3):
-        op._spmv(X)
-    torch.cuda.synchronize()
-    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 20
-    ev0.record(stream)
-    for _ in range(reps):
-        op._spmv(X)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    apply_ms = ev0.elapsed_time(ev1) / reps
+    apply_ms = time_apply(op, X, stream)
     nnz = args.n * k
     apply_bytes = nnz * (4 + 8) + 8 * (args.n + args.n)  # col idx + values + x + y
     apply_gbs = apply_bytes / (apply_ms * 1e-3) / 1e9
+    # the same cloud after reorder_points (the reference's locality utility): X gathers
+    # of neighbouring rows then share cache lines
+    q = pts.copy()
+    R.reorder_points(q, k, device=dev)
+    d_q = torch.as_tensor(q, device=dev)
+    op_q = R.RadialBasisOperator(R.Laplacian(), d_q, basis=R.PHS(3, poly_deg=deg), k=k)
+    fq, _ = L.test_field(q)
+    apply_q_ms = time_apply(op_q, torch.as_tensor(fq[:, None], device=dev).contiguous(), stream)
+    apply_q_gbs = apply_bytes / (apply_q_ms * 1e-3) / 1e9
+    del op_q, d_q
 
     plain = results["fp64"]
     lr = results["learned"]
@@ -149,10 +252,12 @@ def main():
     n_acc = lr["status"]["learned"]
     n_rej = args.n - n_acc
     if st[0] >= st[1] + st[2]:
-        dom = {"kernel": "k_learned (surrogate + FP64 acceptance)", "bound": "mfma", "unit": "TFLOP/s",
+        dom = {"kernel": "k_learned (surrogate + FP64 acceptance)", "trace_name": "k_learned",
+               "bound": "mfma", "unit": "TFLOP/s",
                "peak": BF16_PEAK_TFLOPS, "per_unit": mlp_flops(), "units": args.n, "ms": st[0]}
     else:
-        dom = {"kernel": "k_build<float> (FP32 fallback solve)", "bound": "valu-fp32 (unpacked: FP64 rate)", "unit": "TFLOP/s",
+        dom = {"kernel": "k_build<float> (FP32 fallback solve)", "trace_name": "k_build<float,32,2,1>",
+               "bound": "valu-fp32 (unpacked: FP64 rate)", "unit": "TFLOP/s",
                "peak": FP64_PEAK_TFLOPS, "per_unit": flops_per_stencil(n_sys, 1), "units": n_rej,
                "ms": st[1]}
     achieved = dom["per_unit"] * dom["units"] / (dom["ms"] * 1e-3) / 1e12
@@ -198,13 +303,19 @@ def main():
         "model": model_info,
         "knn_ms": knn_s * 1e3,
         "apply": {"ms": apply_ms, "GB_s": apply_gbs, "frac_hbm_peak": apply_gbs / HBM_PEAK_GBS,
-                  "bytes": apply_bytes},
+                  "bytes": apply_bytes, "traffic": profiled_traffic("k_spmv_staged:cloud"),
+                  "kernel": "k_spmv_staged (rows in Morton order, staged through LDS)",
+                  "reordered_cloud": {"ms": apply_q_ms, "GB_s": apply_q_gbs,
+                                      "frac_hbm_peak": apply_q_gbs / HBM_PEAK_GBS,
+                                      "traffic": profiled_traffic("k_spmv_staged:reordered")}},
         "roofline": {"bound": dom["bound"], "achieved": achieved, "peak": dom["peak"], "unit": dom["unit"],
-                     "frac": achieved / dom["peak"], "traffic": None, "kernel": dom["kernel"],
+                     "frac": achieved / dom["peak"], "traffic": profiled_traffic(dom["trace_name"]),
+                     "kernel": dom["kernel"],
                      "per_unit_flops": dom["per_unit"], "units_per_launch": dom["units"],
                      "kernel_ms": dom["ms"]},
         "plain_fp64_roofline": {"bound": "valu-fp64", "achieved": fp64_achieved, "peak": FP64_PEAK_TFLOPS,
                                 "unit": "TFLOP/s", "frac": fp64_achieved / FP64_PEAK_TFLOPS,
+                                "traffic": profiled_traffic("k_build<double,32,2,1>"),
                                 "per_unit_flops": flops_per_stencil(n_sys, 1), "kernel_ms": stage_ms["fp64"][0]},
         "cpu_baseline": cpu,
     }

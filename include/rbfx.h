@@ -153,6 +153,11 @@ typedef struct rbfx_build_args {
                                     of the radius-scaled system (the reference keeps
                                     only the first k rows, assembly.jl:225-226)      */
     const struct rbfx_mlp_model* model;  /* RBFX_PREC_LEARNED: host struct          */
+    const int32_t* order;        /* optional n_eval: the order in which stencils are
+                                    solved (a permutation, e.g. rbfx_spatial_order of
+                                    the eval points), so neighbouring wavefronts gather
+                                    neighbouring coordinates; results do not depend on
+                                    it. Entries outside 0..n_eval-1 are skipped.      */
 } rbfx_build_args;
 
 /* Surrogate for RBFX_PREC_LEARNED (no reference counterpart). A BF16 MLP

@@ -14,6 +14,7 @@ from .errors import ArgumentError, DimensionMismatch, SingularException
 from .operators import (AbstractOperator, RadialBasisOperator, Laplacian, Partial, MixedPartial, Jacobian, Hessian,
                         Regrid, Identity, ScaledOperator, SumOperator, Custom, Hermite, laplacian, partial,
                         mixed_partial, jacobian, gradient, hessian, regrid, custom, find_neighbors, autoselect_k,
+                        reorder_points,
                         weights, update_weights, is_cache_valid, print_op)
 from .learned import LearnedModel
 

@@ -280,6 +280,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
     P.eval = a->eval;
     P.adj_ptr = a->adj_ptr;
     P.adj_idx = a->adj_idx;
+    P.order = a->order;
     P.k = a->k;
     P.basis_kind = kind;
     P.eps = a->basis.eps;
@@ -347,6 +348,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
             Q.data = a->data;
             Q.eval = a->eval;
             Q.adj_idx = a->adj_idx;
+            Q.order = a->order;
             Q.k = a->k;
             Q.basis_kind = kind;
             Q.eps = a->basis.eps;

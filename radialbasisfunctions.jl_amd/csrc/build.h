@@ -7,6 +7,19 @@
 
 namespace rbfx {
 
+// Workgroups are dealt to the 8 XCDs round-robin. With a grid padded to a multiple of
+// 8, this maps them so each XCD walks one contiguous eighth of the work (in spatial
+// order, its L2 then serves the coordinate gathers of neighbouring stencils).
+// Returns -1 for the padding blocks.
+__device__ __forceinline__ int64_t xcd_contiguous_block(int64_t nb_real)
+{
+    const int64_t nb = gridDim.x;
+    const int64_t b = blockIdx.x;
+    const int64_t lb = (b & 7) * (nb >> 3) + (b >> 3);
+    return lb < nb_real ? lb : -1;
+}
+__host__ __device__ inline int64_t pad_to_xcds(int64_t nb) { return (nb + 7) / 8 * 8; }
+
 struct TermDev {
     int col, kind, d1, d2;
     double coeff;
@@ -38,6 +51,8 @@ struct BuildDev {
     int64_t out_nnz;
     double* out_lambda;  // optional: n_cols * n_eval * n_mono multipliers
     int32_t* status;
+    // optional processing order of all n_eval stencils (used without a work list)
+    const int32_t* order;
     // optional work list (stencil ids) whose length lives in device memory
     const int32_t* list;
     const int32_t* list_len;
@@ -47,6 +62,7 @@ struct BuildDev {
     int refine_steps;
     double accept_tol;
     int ok_status;  // status written for a stencil solved by this launch
+    int64_t n_blocks;  // workgroups with work (set by the launcher; the grid is padded)
 };
 
 // largest supported polynomial degree (any dimension), and the exponent table

@@ -313,10 +313,14 @@ __device__ __forceinline__ void axpy_row(T (&r)[N], T a, const T* u, int n_cols)
         if (c < n_cols) r[NMAX + c] = fma(a, u[NMAX + c], r[NMAX + c]);
 }
 
-// waves per SIMD the register allocation targets
+// waves per SIMD the register allocation targets (FP64, two stencils per wavefront,
+// up to 4 right-hand sides: a compile-time tuning constant)
+#ifndef RBFX_FP64_G2_WAVES
+#define RBFX_FP64_G2_WAVES 3
+#endif
 template <typename T, int NMAX, int RC>
 struct BuildOccupancy {
-    static constexpr int waves = sizeof(T) == 4 ? 2 : (NMAX <= 32 ? (RC <= 4 ? 3 : 2) : 2);
+    static constexpr int waves = sizeof(T) == 4 ? 2 : (NMAX <= 32 ? (RC <= 4 ? RBFX_FP64_G2_WAVES : 2) : 2);
 };
 
 // ------------------------------------------------------------------ kernel
@@ -348,9 +352,15 @@ k_build(const BuildDev P)
         if (L < RBFX_MAX_TERMS) s_terms[L] = P.terms[L];
     }
 
-    const int64_t wi = (int64_t)blockIdx.x * G + g;
+    const int64_t lb = xcd_contiguous_block(P.n_blocks);
+    if (lb < 0) return;  // padding block: the whole workgroup leaves
+    const int64_t wi = lb * G + g;
     bool alive = wi < n_work;
-    const int64_t e = alive ? (P.list ? (int64_t)P.list[wi] : wi) : 0;
+    int64_t e = alive ? (P.list ? (int64_t)P.list[wi] : (P.order ? (int64_t)P.order[wi] : wi)) : 0;
+    if (e < 0 || e >= P.n_eval) {  // an order entry out of range: nothing to solve
+        alive = false;
+        e = 0;
+    }
     const int64_t s0 = alive ? (P.adj_ptr ? P.adj_ptr[e] : e * (int64_t)P.k) : 0;
     const int kk = alive ? (P.adj_ptr ? (int)(P.adj_ptr[e + 1] - s0) : P.k) : 0;
     const int64_t o0 = alive ? (P.out_ptr ? P.out_ptr[e] : e * (int64_t)P.k) : 0;
@@ -767,8 +777,9 @@ inline hipError_t launch_one(const BuildDev& P, int64_t n_work, hipStream_t s)
     if (n_work <= 0) return hipSuccess;
     constexpr int G = 64 / NMAX;
     // one group of NMAX lanes per stencil; a work-list launch sizes the grid by its upper bound
-    const int64_t grid = (n_work + G - 1) / G;
-    hipLaunchKernelGGL((k_build<T, NMAX, DIM, RC>), dim3((unsigned)grid), dim3(64), 0, s, P);
+    BuildDev Q = P;
+    Q.n_blocks = (n_work + G - 1) / G;
+    hipLaunchKernelGGL((k_build<T, NMAX, DIM, RC>), dim3((unsigned)pad_to_xcds(Q.n_blocks)), dim3(64), 0, s, Q);
     return hipGetLastError();
 }
 

@@ -30,6 +30,7 @@ struct LearnedDev {
     const double* data;
     const double* eval;
     const int32_t* adj_idx;  // n_eval * k (fixed stencil size)
+    const int32_t* order;    // optional processing order of the stencils
     int k;
     int basis_kind;
     double eps;
@@ -51,6 +52,7 @@ struct LearnedDev {
     int32_t* reject_list;
     int32_t* reject_count;
     double* out_rho;  // optional per-stencil relative residual
+    int64_t n_blocks;  // workgroups with work (set by the launcher; the grid is padded)
 };
 
 hipError_t launch_learned_build(const LearnedDev& P, hipStream_t s);

@@ -112,8 +112,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
 
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int sl = t >> 2, q = t & 3;
-    const int64_t e = (int64_t)blockIdx.x * kBM + sl;
-    const bool live = e < P.n_eval;
+    const int64_t lb = xcd_contiguous_block(P.n_blocks);
+    if (lb < 0) return;  // padding block: the whole workgroup leaves
+    const int64_t wi = lb * kBM + sl;
+    int64_t e = (P.order && wi < P.n_eval) ? (int64_t)P.order[wi] : wi;
+    const bool live = wi < P.n_eval && e >= 0 && e < P.n_eval;
+    if (!live) e = 0;
     if (t < MM * 3) (&s_mono[0][0])[t] = (&P.mono_e[0][0])[t];
     if (t < RBFX_MAX_TERMS) s_terms[t] = P.terms[t];
 
@@ -374,8 +378,9 @@ hipError_t launch_learned_build(const LearnedDev& P, hipStream_t s)
 {
     if (P.n_eval <= 0) return hipSuccess;
     if (P.k != 20 || P.n_mono != 6) return hipErrorInvalidValue;
-    const int64_t grid = (P.n_eval + kBM - 1) / kBM;
-    hipLaunchKernelGGL((k_learned<20, 6>), dim3((unsigned)grid), dim3(256), 0, s, P);
+    LearnedDev Q = P;
+    Q.n_blocks = (P.n_eval + kBM - 1) / kBM;
+    hipLaunchKernelGGL((k_learned<20, 6>), dim3((unsigned)pad_to_xcds(Q.n_blocks)), dim3(256), 0, s, Q);
     return hipGetLastError();
 }
 

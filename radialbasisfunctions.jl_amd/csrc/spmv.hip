@@ -20,6 +20,9 @@
 
 namespace rbfx {
 
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+
 // Workgroups are dealt to the 8 XCDs round-robin; remap so each XCD walks one
 // contiguous range of (spatially ordered) rows and its own L2 serves their gathers.
 // The grid is padded to a multiple of 8; blocks mapped past the end return -1.
@@ -205,10 +208,29 @@ __global__ __launch_bounds__(256) void k_spmv_staged(const SpmvDev P, int64_t nb
         rows_gather_reduce<NV, G, FMAX, U>(P, r0, r1, 0, P.col_idx, P.vals, P.nnz);
         return;
     }
-    for (int i = threadIdx.x; i < n_e; i += 256) {
-        s_col[i] = __builtin_nontemporal_load(P.col_idx + b0 + i);
+    // 16-byte loads (2 values / 4 indices per lane) where the run and both copies are
+    // 16-byte aligned, else one element per lane
+    bool vec = (n_e & 3) == 0 && (cap & 1) == 0 && ((uintptr_t)(P.col_idx + b0) & 15) == 0;
 #pragma unroll
-        for (int v = 0; v < NV; ++v) s_val[v * cap + i] = __builtin_nontemporal_load(P.vals + (int64_t)v * P.nnz + b0 + i);
+    for (int v = 0; v < NV; ++v) vec = vec && ((uintptr_t)(P.vals + (int64_t)v * P.nnz + b0) & 15) == 0;
+    if (vec) {
+        const int n4 = (int)(n_e >> 2), n2 = (int)(n_e >> 1);
+        const i32x4* c4 = reinterpret_cast<const i32x4*>(P.col_idx + b0);
+        i32x4* sc4 = reinterpret_cast<i32x4*>(s_col);
+        for (int i = threadIdx.x; i < n4; i += 256) sc4[i] = __builtin_nontemporal_load(c4 + i);
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const f64x2* v2 = reinterpret_cast<const f64x2*>(P.vals + (int64_t)v * P.nnz + b0);
+            f64x2* sv2 = reinterpret_cast<f64x2*>(s_val + (int64_t)v * cap);
+            for (int i = threadIdx.x; i < n2; i += 256) sv2[i] = __builtin_nontemporal_load(v2 + i);
+        }
+    } else {
+        for (int i = threadIdx.x; i < n_e; i += 256) {
+            s_col[i] = __builtin_nontemporal_load(P.col_idx + b0 + i);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                s_val[v * cap + i] = __builtin_nontemporal_load(P.vals + (int64_t)v * P.nnz + b0 + i);
+        }
     }
     __syncthreads();
     rows_gather_reduce<NV, G, FMAX, U>(P, r0, r1, b0, (const int32_t*)s_col, (const double*)s_val, (int64_t)cap);

@@ -277,6 +277,39 @@ def find_neighbors(data, eval_points=None, k=None, device=None):
     return _lib.knn(X, Q, int(k)).cpu().numpy()
 
 
+def reorder_points(x, *args, device=None):
+    """reorder_points!(x, k) / reorder_points!(x, adjl, k) (utils.jl:45-55): permute the
+    points of x in place into a locality-preserving order and return the permutation
+    perm (0-based int64, new x = old x[perm]).
+
+    The reference orders by reverse Cuthill-McKee on the k-NN graph; here the order is
+    a Z-order (Morton) curve of the coordinates, computed on the GPU, which serves the
+    same purpose: points that are close get nearby indices, so the X gathers of
+    neighbouring stencils share cache lines. adjl and k keep the reference's call
+    forms; the curve does not need them. x is a numpy array or torch tensor (N, d)."""
+    if len(args) == 1:
+        adjl, k = None, args[0]
+    elif len(args) == 2:
+        adjl, k = args
+    else:
+        raise TypeError("reorder_points(x, k) or reorder_points(x, adjl, k)")
+    if isinstance(k, bool) or not isinstance(k, (int, np.integer)):
+        raise TypeError(f"k must be an integer (got {type(k).__name__})")
+    t = _lib.torch()
+    is_torch = isinstance(x, t.Tensor)
+    dev = x.device if (is_torch and x.is_cuda) else _lib.device_for(device)
+    pts = _to_device(_as_points(x), dev)
+    if adjl is not None and len(adjl) != pts.shape[0]:
+        raise DimensionMismatch(f"adjl has {len(adjl)} entries, x has {pts.shape[0]} points")
+    perm = _lib.spatial_order(pts).to(t.int64)
+    if is_torch:
+        x.copy_(x[perm.to(x.device)])
+        return perm.cpu().numpy()
+    perm = perm.cpu().numpy()
+    x[...] = np.asarray(x)[perm]
+    return perm
+
+
 class Hermite:
     """Hermite boundary data: is_boundary (N,), bc (n_bnd BoundaryConditions),
     normals (n_bnd, d) -- the `hermite` NamedTuple of operators.jl:92-93."""
@@ -476,6 +509,12 @@ class RadialBasisOperator:
             mstruct = self.model.struct(self.model._rho)
             keep.append(mstruct)
             a.model = C.addressof(mstruct)
+        if self._row_order is None and self.n_eval >= self.SPATIAL_ORDER_MIN:
+            # stencils solved (and rows applied) along a Z-curve of the eval points, so
+            # neighbouring wavefronts gather neighbouring coordinates; results do not
+            # depend on the order
+            self._row_order = _lib.spatial_order(self._d_eval)
+        a.order = _lib.ptr(self._row_order)
         self._args, self._keep = a, keep
         rc, counts = _lib.build_weights(a, self.dev)
         self.status_counts = counts

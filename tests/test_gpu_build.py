@@ -335,10 +335,9 @@ def test_spatial_order_and_ordered_apply(RBF, gpu):
     step = np.sqrt(((pts[order[1:]] - pts[order[:-1]]) ** 2).sum(1))
     assert np.median(step) < 0.05 * np.median(np.sqrt(((pts[1:] - pts[:-1]) ** 2).sum(1)))
     op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
-    assert op._row_order is None
+    assert op._row_order is not None  # n_eval above the threshold: ordered build and apply
     f = torch.as_tensor(np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1]), device=gpu)
     y1 = op(f).cpu().numpy()
-    assert op._row_order is not None  # n_eval above the threshold: ordered apply
     keep = op._row_order
     op._row_order = None
     op.SPATIAL_ORDER_MIN = 10 ** 12
@@ -353,6 +352,40 @@ def test_spatial_order_and_ordered_apply(RBF, gpu):
     for c in range(2):
         ref = W[c] @ F.cpu().numpy()
         assert np.allclose(G1[..., c], ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_reorder_points(RBF, gpu):
+    """reorder_points (utils.jl:45-55) in both call forms, as test/utils.jl:61-77 checks
+    them: the result is a permutation and the points are permuted in place
+    (points == original[perm]). Beyond that: on a random cloud the new order keeps
+    neighbours close, a torch tensor is permuted in place too, and k must be an integer."""
+    import torch
+
+    original = np.array([[float(i), 0.5 * i] for i in range(10, 0, -1)])
+    pts = original.copy()
+    perm = RBF.reorder_points(pts, 3, device=gpu)
+    assert np.array_equal(np.sort(perm), np.arange(10))
+    assert np.array_equal(pts, original[perm])
+    pts = original.copy()
+    adjl = RBF.find_neighbors(pts, k=3, device=gpu)
+    perm3 = RBF.reorder_points(pts, adjl, 3, device=gpu)
+    assert np.array_equal(np.sort(perm3), np.arange(10))
+    assert np.array_equal(pts, original[perm3])
+    with pytest.raises(TypeError):
+        RBF.reorder_points(pts, 3.0, device=gpu)
+
+    rng = np.random.default_rng(11)
+    cloud = rng.random((50_000, 2))
+    q = cloud.copy()
+    perm = RBF.reorder_points(q, 20, device=gpu)
+    assert np.array_equal(np.sort(perm), np.arange(len(cloud)))
+    assert np.array_equal(q, cloud[perm])
+    step = np.sqrt(((q[1:] - q[:-1]) ** 2).sum(1))
+    step0 = np.sqrt(((cloud[1:] - cloud[:-1]) ** 2).sum(1))
+    assert np.median(step) < 0.05 * np.median(step0)
+    qt = torch.as_tensor(cloud, device=gpu)
+    perm_t = RBF.reorder_points(qt, 20)
+    assert np.array_equal(qt.cpu().numpy(), cloud[perm_t])
 
 
 @pytest.mark.parametrize("fields", [1, 3, 6])
