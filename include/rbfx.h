@@ -88,12 +88,15 @@ enum {
     RBFX_ST_SINGULAR = 8,        /* zero pivot: system singular                     */
     RBFX_ST_EVAL_NOT_FOUND = 9,  /* Hermite stencil without its eval point
                                     (assembly.jl:35-46 raises ArgumentError)        */
-    RBFX_ST_TOO_LARGE = 10       /* n = k + m beyond the supported maximum          */
+    RBFX_ST_TOO_LARGE = 10,      /* n = k + m beyond the supported maximum          */
+    RBFX_ST_INVALID = 11         /* per-stencil input out of range (degree map)     */
 };
 
 #define RBFX_MAX_TERMS 16
 #define RBFX_MAX_COLS 8
-#define RBFX_MAX_N 64            /* largest k + m one wavefront factorises */
+#define RBFX_MAX_N 80            /* largest k + m: up to 64 rows factorised in one
+                                    wavefront's registers, 65..80 with the system in
+                                    LDS (within 64 KB at up to 8 operator columns)   */
 
 typedef struct rbfx_basis {
     int32_t kind;        /* RBFX_PHS* / RBFX_IMQ / RBFX_GAUSSIAN             */
@@ -153,6 +156,9 @@ typedef struct rbfx_build_args {
                                     of the radius-scaled system (the reference keeps
                                     only the first k rows, assembly.jl:225-226)      */
     const struct rbfx_mlp_model* model;  /* RBFX_PREC_LEARNED: host struct          */
+    const int8_t* poly_deg_map;  /* optional n_eval: polynomial degree of each stencil
+                                    (-1..basis.poly_deg; no reference counterpart: one
+                                    operator mixing degrees, e.g. 3 near a boundary) */
     const int32_t* order;        /* optional n_eval: the order in which stencils are
                                     solved (a permutation, e.g. rbfx_spatial_order of
                                     the eval points), so neighbouring wavefronts gather
@@ -293,6 +299,18 @@ int rbfx_last_build_counts(int64_t counts[16]);
  * FP64 re-solve of non-converged stencils, [3] the whole build. No reference
  * counterpart (instrumentation).                                                  */
 int rbfx_last_build_timing(double ms[4]);
+
+/* Size buckets of the last build on this thread (no reference counterpart): stencils
+ * with n = k + m <= 32, 33..64 and 65..RBFX_MAX_N, and the kernel each ran on
+ * (0: registers, two stencils per wavefront; 1: registers, one; 2: system in LDS;
+ * -1: none). Uneven builds (ragged k or a degree map) classify stencils into these
+ * buckets; the kernel of a bucket is timed on a sample the first time a
+ * configuration is seen, and the fastest is kept (counts of -1: not read back).  */
+int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3]);
+
+/* Pin the kernel of a size bucket for later builds in this process (kernel -1:
+ * automatic choice again). For tests and tuning; no reference counterpart.      */
+int rbfx_tune_bucket(int32_t bucket, int32_t kernel);
 
 /* ---- surrogate training and inputs (no reference counterpart) ------------ */
 

@@ -128,6 +128,7 @@ class BuildArgs(C.Structure):
         ("stream", C.c_void_p),
         ("out_lambda", C.c_void_p),
         ("model", C.c_void_p),
+        ("poly_deg_map", C.c_void_p),
         ("order", C.c_void_p),
     ]
 

@@ -87,7 +87,7 @@ def exported_symbols():
             "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
             "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async",
-            "rbfx_permute_rows"]
+            "rbfx_permute_rows", "rbfx_last_build_buckets", "rbfx_tune_bucket"]
 
 
 def _check(rc):
@@ -149,6 +149,20 @@ def build_weights(args, dev):
     counts = (C.c_int64 * 16)()
     lib().rbfx_last_build_counts(counts)
     return rc, list(counts)
+
+
+def last_build_buckets():
+    """(stencils per size bucket, kernel per bucket) of the last build on this thread:
+    buckets n <= 32, 33..64, 65..; kernels 0 registers x2, 1 registers, 2 LDS, -1 none."""
+    n = (C.c_int64 * 3)()
+    k = (C.c_int32 * 3)()
+    _check(lib().rbfx_last_build_buckets(n, k))
+    return list(n), list(k)
+
+
+def tune_bucket(bucket, kernel):
+    """Pin the kernel of a size bucket (-1: automatic)."""
+    _check(lib().rbfx_tune_bucket(int(bucket), int(kernel)))
 
 
 def spmv(args, dev, asynchronous=False):

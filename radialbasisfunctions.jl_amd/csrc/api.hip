@@ -10,6 +10,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -58,6 +59,12 @@ struct DeviceState {
     size_t cap_reject = 0;
     int32_t* reject_count = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};  // stage boundaries of a build
+    // size buckets of an uneven build: three lists of stencil ids (n_eval each) + counts
+    int32_t* bucket_list = nullptr;
+    size_t cap_bucket = 0;
+    int32_t* bucket_count = nullptr;  // [3] list lengths, [3..5] tuning sample sizes
+    hipEvent_t tev[2] = {nullptr, nullptr};
+    std::map<int, int> tuned;  // (dim, columns, precision, bucket) -> kernel
 };
 
 std::mutex g_mu;
@@ -98,6 +105,39 @@ __global__ void k_zero_u64(unsigned long long* p, int n)
 }
 
 __global__ void k_zero_i32(int32_t* p) { *p = 0; }
+__global__ void k_set_i32(int32_t* p, int32_t v) { *p = v; }
+
+// Size buckets of an uneven build, in processing order: n = k + m <= 32, <= 64, above.
+// Lanes of a wavefront append with one atomic per bucket, so each list keeps the
+// (spatial) order of the stencils at wavefront granularity.
+__global__ __launch_bounds__(256) void k_classify(const rbfx::BuildDev P, int dim, int32_t* lists, int32_t* counts)
+{
+    const int64_t wi = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    int b = -1;
+    int32_t e = 0;
+    if (wi < P.n_eval) {
+        e = P.order ? P.order[wi] : (int32_t)wi;
+        if (e >= 0 && e < P.n_eval) {
+            const int kk = P.adj_ptr ? (int)(P.adj_ptr[e + 1] - P.adj_ptr[e]) : P.k;
+            const int m = rbfx::stencil_n_mono(P, e, dim);
+            const int nn = kk + (m < 0 ? 0 : m);
+            b = nn <= 32 ? 0 : (nn <= 64 ? 1 : 2);
+        }
+    }
+    const int lane = threadIdx.x & 63;
+    for (int bb = 0; bb < 3; ++bb) {
+        const unsigned long long mask = __ballot(b == bb);
+        if (!mask) continue;
+        const int leader = __builtin_ctzll(mask);
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&counts[bb], __popcll(mask));
+        base = __shfl(base, leader, 64);
+        if (b == bb) {
+            const int rank = __popcll(mask & ((1ull << lane) - 1ull));
+            lists[(int64_t)bb * P.n_eval + base + rank] = e;
+        }
+    }
+}
 
 int n_mono_host(int dim, int deg)
 {
@@ -171,6 +211,135 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
 }
 
 
+// ---------------------------------------------------------------- size buckets
+// Kernels a bucket can run on: the register kernel with two stencils per wavefront
+// (n <= 32), the register kernel with one (n <= 64), the LDS kernel (n <= RBFX_MAX_N).
+enum { K_REG32 = 0, K_REG64 = 1, K_WIDE = 2 };
+int g_force_kernel[3] = {-1, -1, -1};  // rbfx_tune_bucket overrides
+thread_local int64_t g_bucket_n[3];
+thread_local int32_t g_bucket_kernel[3];
+constexpr int64_t kTuneMin = 4096;     // stencils in a bucket before its mapping is timed
+constexpr int32_t kTuneSample = 16384;
+
+static bool kernel_serves(int kid, int bucket, bool fp32, int n_cols)
+{
+    if (kid == K_REG32) return bucket == 0;
+    if (kid == K_REG64) return bucket <= 1;
+    if (kid == K_WIDE) return bucket == 2 || (!fp32 && bucket == 1 && rbfx::wide_fits(64, n_cols));
+    return false;
+}
+
+static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, int dim, bool fp32, int rows_cap, int64_t n_work,
+                                   hipStream_t st)
+{
+    if (kid == K_WIDE) {
+        rbfx::BuildDev Q = P;
+        if (fp32) Q.ok_status = RBFX_ST_FALLBACK64;  // the LDS kernel solves in FP64
+        return rbfx::launch_build_wide(Q, dim, rows_cap, n_work, st);
+    }
+    return rbfx::launch_build(P, dim, kid == K_REG32 ? 32 : 64, fp32, n_work, st);
+}
+
+// Main solve of all stencils (plain FP64, or FP32 + refinement). Stencils of one size
+// run in one launch. Uneven stencils (ragged k, per-stencil degree) are classified into
+// size buckets in processing order, and each bucket runs on the kernel chosen for it:
+// forced (rbfx_tune_bucket), or timed once per configuration on a sample of the bucket
+// (the autotuner), or the narrowest kernel that fits.
+static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceState& ds, int nmax, bool fp32,
+                     bool async, hipStream_t st)
+{
+    const int dim = a->dim;
+    for (int b = 0; b < 3; ++b) {
+        g_bucket_n[b] = 0;
+        g_bucket_kernel[b] = -1;
+    }
+    const bool uneven = (a->adj_ptr || a->poly_deg_map) && nmax > 32;
+    if (!uneven) {
+        const int kid = nmax <= 32 ? K_REG32 : (nmax <= 64 ? K_REG64 : K_WIDE);
+        const int b = nmax <= 32 ? 0 : (nmax <= 64 ? 1 : 2);
+        g_bucket_n[b] = a->n_eval;
+        g_bucket_kernel[b] = kid;
+        if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
+        HIPCHK(launch_kernel_id(kid, P, dim, fp32, nmax, a->n_eval, st));
+        return RBFX_OK;
+    }
+    const size_t need = sizeof(int32_t) * 3 * (size_t)a->n_eval;
+    if (!async) {
+        HIPCHK(grow((void**)&ds.bucket_list, ds.cap_bucket, need));
+        if (!ds.bucket_count) HIPCHK(hipMalloc((void**)&ds.bucket_count, 6 * sizeof(int32_t)));
+    } else if (ds.cap_bucket < need || !ds.bucket_count) {
+        return set_err(RBFX_EINVAL, "asynchronous build: run one blocking build with these sizes first");
+    }
+    for (int b = 0; b < 3; ++b) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.bucket_count + b);
+    hipLaunchKernelGGL(k_classify, dim3((unsigned)((a->n_eval + 255) / 256)), dim3(256), 0, st, P, dim,
+                       ds.bucket_list, ds.bucket_count);
+    HIPCHK(hipGetLastError());
+    int32_t cnt[3] = {(int32_t)a->n_eval, (int32_t)a->n_eval, (int32_t)a->n_eval};
+    if (!async) {
+        HIPCHK(hipMemcpyAsync(cnt, ds.bucket_count, sizeof(cnt), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
+    const int rows_cap[3] = {32, 64, std::min(nmax, RBFX_MAX_N)};
+    int kid[3];
+    for (int b = 0; b < 3; ++b) {
+        const int dflt = b == 0 ? K_REG32 : (b == 1 ? K_REG64 : K_WIDE);
+        kid[b] = dflt;
+        if (g_force_kernel[b] >= 0 && kernel_serves(g_force_kernel[b], b, fp32, P.n_cols)) {
+            kid[b] = g_force_kernel[b];
+            continue;
+        }
+        const int key = ((dim * 16 + P.n_cols) * 4 + a->precision) * 4 + b;
+        auto it = ds.tuned.find(key);
+        if (it != ds.tuned.end()) {
+            kid[b] = it->second;
+            continue;
+        }
+        if (async || cnt[b] < kTuneMin) continue;
+        // time each kernel that serves this bucket on a sample of its list
+        int cands[3], nc = 0;
+        for (int c = 0; c < 3; ++c)
+            if (kernel_serves(c, b, fp32, P.n_cols)) cands[nc++] = c;
+        if (nc < 2) {
+            ds.tuned[key] = dflt;
+            continue;
+        }
+        const int32_t sample = std::min<int32_t>(cnt[b], kTuneSample);
+        hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, st, ds.bucket_count + 3 + b, sample);
+        rbfx::BuildDev Q = P;
+        Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
+        Q.list_len = ds.bucket_count + 3 + b;
+        for (int i = 0; i < 2; ++i)
+            if (!ds.tev[i]) HIPCHK(hipEventCreate(&ds.tev[i]));
+        float best = 0.f;
+        for (int c = 0; c < nc; ++c) {
+            HIPCHK(launch_kernel_id(cands[c], Q, dim, fp32, rows_cap[b], sample, st));  // warm-up
+            HIPCHK(hipEventRecord(ds.tev[0], st));
+            HIPCHK(launch_kernel_id(cands[c], Q, dim, fp32, rows_cap[b], sample, st));
+            HIPCHK(hipEventRecord(ds.tev[1], st));
+            HIPCHK(hipEventSynchronize(ds.tev[1]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ds.tev[0], ds.tev[1]));
+            if (c == 0 || ms < best) {
+                best = ms;
+                kid[b] = cands[c];
+            }
+        }
+        ds.tuned[key] = kid[b];
+    }
+    // the real launches (a tuning sample's failures are discarded with the old count)
+    if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
+    for (int b = 0; b < 3; ++b) {
+        g_bucket_n[b] = async ? -1 : cnt[b];
+        g_bucket_kernel[b] = kid[b];
+        if (cnt[b] <= 0) continue;
+        rbfx::BuildDev Q = P;
+        Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
+        Q.list_len = ds.bucket_count + b;
+        HIPCHK(launch_kernel_id(kid[b], Q, dim, fp32, rows_cap[b], cnt[b], st));
+    }
+    return RBFX_OK;
+}
+
 static int build_impl(const rbfx_build_args* a, bool async)
 {
     if (!a) return set_err(RBFX_EINVAL, "rbfx_build_weights: null args");
@@ -220,6 +389,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
         if (a->basis.kind != mdl->basis_kind || a->basis.poly_deg != mdl->poly_deg || mdl->poly_deg != 2)
             return set_err(RBFX_EUNSUPPORTED, "learned build: basis differs from the model's");
         if (a->is_boundary) return set_err(RBFX_EUNSUPPORTED, "learned build: Hermite stencils are not covered");
+        if (a->poly_deg_map) return set_err(RBFX_EUNSUPPORTED, "learned build: one polynomial degree only");
         if (a->op.n_cols != 1) return set_err(RBFX_EUNSUPPORTED, "learned build: one operator column only");
         for (int t = 0; t < a->op.n_terms; ++t)
             if (a->op.terms[t].kind != mdl->op_kind || mdl->op_kind != RBFX_OP_LAP)
@@ -295,6 +465,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
     }
     P.poly_deg = deg;
     P.n_mono = deg >= 0 ? rbfx::fill_mono_table(a->dim, deg, P.mono_e) : 0;
+    P.deg_map = a->poly_deg_map;
     P.is_boundary = a->is_boundary;
     P.bnd_index = a->bnd_index;
     P.bc_alpha = a->bc_alpha;
@@ -323,7 +494,8 @@ static int build_impl(const rbfx_build_args* a, bool async)
     };
     HIPCHK(mark(0));
     if (a->precision == RBFX_PREC_FP64) {
-        HIPCHK(rbfx::launch_build(P, a->dim, nmax, false, a->n_eval, st));
+        const int rc = solve_all(a, P, ds, nmax, false, async, st);
+        if (rc != RBFX_OK) return rc;
         HIPCHK(mark(1));
         HIPCHK(mark(2));
     } else {
@@ -378,7 +550,12 @@ static int build_impl(const rbfx_build_args* a, bool async)
                 n_solve = nrej;
             }  // asynchronous: the grid covers every stencil, the kernel reads the count
         }
-        if (n_solve > 0) HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, n_solve, st));
+        if (a->precision == RBFX_PREC_LEARNED) {
+            if (n_solve > 0) HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, n_solve, st));
+        } else {
+            const int rc = solve_all(a, P, ds, nmax, true, async, st);
+            if (rc != RBFX_OK) return rc;
+        }
         // FP32_REFINED: the FP32 solve is the main kernel (stage 1 stays empty)
         if (a->precision != RBFX_PREC_LEARNED) HIPCHK(mark(1));
         HIPCHK(mark(2));
@@ -391,7 +568,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
         F.list = ds.fail_list;
         F.list_len = ds.fail_count;
         F.ok_status = RBFX_ST_FALLBACK64;
-        if (nfail > 0) HIPCHK(rbfx::launch_build(F, a->dim, nmax, false, nfail, st));
+        if (nfail > 0) HIPCHK(rbfx::launch_build(F, a->dim, std::min(nmax, 64), false, nfail, st));
     }
     HIPCHK(mark(3));
     if (async) {
@@ -416,6 +593,9 @@ static int build_impl(const rbfx_build_args* a, bool async)
         g_stage_ms[3] = ms;
     }
     for (int i = 0; i < 16; ++i) g_counts[i] = (int64_t)h[i];
+    if (h[RBFX_ST_INVALID])
+        return set_err(RBFX_EINVAL, "%llu stencils have a polynomial degree outside -1..%d", h[RBFX_ST_INVALID],
+                       a->basis.poly_deg);
     if (h[RBFX_ST_TOO_LARGE])
         return set_err(RBFX_EUNSUPPORTED, "%llu stencils exceed the device maximum k+m=%d", h[RBFX_ST_TOO_LARGE], RBFX_MAX_N);
     if (h[RBFX_ST_EVAL_NOT_FOUND])
@@ -451,6 +631,26 @@ int rbfx_init(int32_t device)
 int rbfx_last_build_counts(int64_t counts[16])
 {
     for (int i = 0; i < 16; ++i) counts[i] = g_counts[i];
+    return RBFX_OK;
+}
+
+int rbfx_tune_bucket(int32_t bucket, int32_t kernel)
+{
+    if (bucket < 0 || bucket > 2) return set_err(RBFX_EINVAL, "bucket must be 0, 1 or 2");
+    if (kernel < -1 || kernel > 2 || (kernel >= 0 && !kernel_serves(kernel, bucket, false, 1)))
+        return set_err(RBFX_EINVAL, "kernel %d cannot serve bucket %d", kernel, bucket);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_force_kernel[bucket] = kernel;
+    g_err.clear();
+    return RBFX_OK;
+}
+
+int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3])
+{
+    for (int b = 0; b < 3; ++b) {
+        n[b] = g_bucket_n[b];
+        kernel[b] = g_bucket_kernel[b];
+    }
     return RBFX_OK;
 }
 

@@ -20,6 +20,11 @@ __device__ __forceinline__ int64_t xcd_contiguous_block(int64_t nb_real)
 }
 __host__ __device__ inline int64_t pad_to_xcds(int64_t nb) { return (nb + 7) / 8 * 8; }
 
+// largest supported polynomial degree (any dimension) and the longest monomial
+// table (3-D, degree 4)
+constexpr int kMaxPolyDeg = 4;
+constexpr int kMaxMono = 35;
+
 struct TermDev {
     int col, kind, d1, d2;
     double coeff;
@@ -38,8 +43,10 @@ struct BuildDev {
     int n_cols, n_terms;
     TermDev terms[RBFX_MAX_TERMS];
     int poly_deg;
-    int n_mono;                 // binomial(dim + deg, dim), 0 for deg = -1
-    int mono_e[RBFX_MAX_N][3];  // exponents of each monomial (graded order)
+    int n_mono;               // binomial(dim + deg, dim), 0 for deg = -1
+    int mono_e[kMaxMono][3];  // exponents of each monomial (graded order: the table of a
+                              // lower degree is a prefix)
+    const int8_t* deg_map;    // optional per-stencil degree (-1..poly_deg)
     const uint8_t* is_boundary;
     const int32_t* bnd_index;
     const double* bc_alpha;
@@ -65,9 +72,32 @@ struct BuildDev {
     int64_t n_blocks;  // workgroups with work (set by the launcher; the grid is padded)
 };
 
-// largest supported polynomial degree (any dimension), and the exponent table
-constexpr int kMaxPolyDeg = 4;
 int fill_mono_table(int dim, int deg, int (*e)[3]);
+
+// number of monomials of total degree <= deg in dim variables (0 for deg = -1)
+__host__ __device__ inline int n_mono_of(int dim, int deg)
+{
+    if (deg < 0) return 0;
+    int num = 1, den = 1;
+    for (int i = 1; i <= dim; ++i) {
+        num *= deg + i;
+        den *= i;
+    }
+    return num / den;
+}
+
+// monomial count of stencil e: its own degree when a degree map is given
+__device__ __forceinline__ int stencil_n_mono(const BuildDev& P, int64_t e, int dim)
+{
+    if (!P.deg_map) return P.n_mono;
+    const int d = P.deg_map[e];
+    return (d >= -1 && d <= P.poly_deg) ? n_mono_of(dim, d) : -1;  // -1: invalid degree
+}
+
+// FP64 build of stencils with n = k + m in 65..RBFX_MAX_N rows (system in LDS);
+// rows_cap is the largest n in the launch
+hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n_work, hipStream_t s);
+bool wide_fits(int rows_cap, int n_cols);
 
 hipError_t launch_build(const BuildDev& P, int dim, int nmax, bool fp32, int64_t n_work, hipStream_t s);
 

@@ -336,19 +336,18 @@ k_build(const BuildDev P)
 
     __shared__ StencilLds<T, NMAX, DIM, UW> s_st[G];
     __shared__ double s_x[G][NL];
-    __shared__ int s_mono[RBFX_MAX_N][3];
+    __shared__ int s_mono[kMaxMono][3];
     __shared__ TermDev s_terms[RBFX_MAX_TERMS];
 
     const int L = lane_id();
     const int g = L / NMAX, li = L % NMAX;
     StencilLds<T, NMAX, DIM, UW>& S = s_st[g];
     const int n_cols = P.n_cols;
-    const int M = P.n_mono;
     const int n_terms = P.n_terms;
     const int64_t n_work = P.list ? (int64_t)*P.list_len : P.n_eval;
     {
         const int* src = &P.mono_e[0][0];
-        for (int i = L; i < RBFX_MAX_N * 3; i += 64) (&s_mono[0][0])[i] = src[i];
+        for (int i = L; i < kMaxMono * 3; i += 64) (&s_mono[0][0])[i] = src[i];
         if (L < RBFX_MAX_TERMS) s_terms[L] = P.terms[L];
     }
 
@@ -364,6 +363,12 @@ k_build(const BuildDev P)
     const int64_t s0 = alive ? (P.adj_ptr ? P.adj_ptr[e] : e * (int64_t)P.k) : 0;
     const int kk = alive ? (P.adj_ptr ? (int)(P.adj_ptr[e + 1] - s0) : P.k) : 0;
     const int64_t o0 = alive ? (P.out_ptr ? P.out_ptr[e] : e * (int64_t)P.k) : 0;
+    const int m_e = alive ? stencil_n_mono(P, e, DIM) : P.n_mono;
+    const int M = m_e < 0 ? 0 : m_e;  // monomials of this stencil (its degree)
+    if (alive && m_e < 0) {
+        if (li == 0 && P.status) P.status[e] = RBFX_ST_INVALID;
+        alive = false;
+    }
     if (alive && (kk < 1 || M + kk > NMAX)) {
         if (li == 0 && P.status) P.status[e] = RBFX_ST_TOO_LARGE;
         alive = false;
@@ -726,7 +731,7 @@ k_build(const BuildDev P)
             if (c < n_cols && is_pt)
                 P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = gfail ? __builtin_nan("") : (double)x[c];
             if (c < n_cols && is_poly && P.out_lambda)
-                P.out_lambda[((int64_t)c * P.n_eval + e) * M + li] = gfail ? __builtin_nan("") : (double)x[c];
+                P.out_lambda[((int64_t)c * P.n_eval + e) * P.n_mono + li] = gfail ? __builtin_nan("") : (double)x[c];
         }
     } else {
         // FP64 residual b - A x, FP32 correction on the same factors, per column
@@ -755,7 +760,7 @@ k_build(const BuildDev P)
                 if (!(rel <= P.accept_tol)) converged = false;
                 if (is_pt) P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = gfail ? __builtin_nan("") : x64;
                 if (is_poly && P.out_lambda)
-                    P.out_lambda[((int64_t)c * P.n_eval + e) * M + li] = gfail ? __builtin_nan("") : x64;
+                    P.out_lambda[((int64_t)c * P.n_eval + e) * P.n_mono + li] = gfail ? __builtin_nan("") : x64;
             }
         }
         if (alive) {

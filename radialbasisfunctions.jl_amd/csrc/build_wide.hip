@@ -1,0 +1,476 @@
+// Fused build for stencil systems of 65..RBFX_MAX_N rows (k up to 60 with a degree-3
+// polynomial gives 70): one wavefront per stencil, the augmented system [A | b] in LDS
+// (row-major, leading dimension lda >= n + RC, odd), lane li owning rows li and li+64.
+//
+// The arithmetic is that of k_build (build_kernel.h): centred, radius-scaled
+// saddle-point system assembled in FP64, rook-pivoted LDL^T (pivot tests of Ashcraft,
+// Grimes & Lewis) with rows that never move, the trailing update applied to every
+// column of the still-active rows, and a back substitution over the rows frozen at
+// their pivot step. Only the storage differs: the rows live in LDS instead of
+// registers, so a pivot-row read is an LDS load and the pivot search reads the
+// column entries of the lane's own two rows.
+#include "build_kernel.h"
+
+namespace rbfx {
+
+constexpr int kWideRows = RBFX_MAX_N;  // <= 128: two rows per lane
+static_assert(kWideRows <= 128, "two rows per lane");
+
+__device__ __forceinline__ uint32_t row_key(double v, int row)
+{
+    const uint32_t hi = (uint32_t)((unsigned long long)__double_as_longlong(v) >> 32);
+    return (hi & ~127u) | (uint32_t)(127 - row);
+}
+
+template <int DIM, int RC>
+__global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wide(const BuildDev P, int rows_cap,
+                                                                                  int lda)
+{
+    extern __shared__ __attribute__((aligned(16))) double s_A[];  // rows_cap x lda
+    __shared__ double s_xi[kWideRows][DIM];
+    __shared__ double s_ba[kWideRows], s_bb[kWideRows], s_bn[kWideRows][DIM];
+    __shared__ int s_bt[kWideRows];
+    __shared__ int s_piv[kWideRows], s_blk[kWideRows];
+    __shared__ int s_mono[kMaxMono][3];
+    __shared__ TermDev s_terms[RBFX_MAX_TERMS];
+
+    const int L = threadIdx.x;
+    const int n_cols = P.n_cols, n_terms = P.n_terms;
+    for (int i = L; i < kMaxMono * 3; i += 64) (&s_mono[0][0])[i] = (&P.mono_e[0][0])[i];
+    if (L < RBFX_MAX_TERMS) s_terms[L] = P.terms[L];
+
+    // one stencil per workgroup: every exit below is uniform over the workgroup
+    const int64_t lb = xcd_contiguous_block(P.n_blocks);
+    if (lb < 0) return;
+    const int64_t n_work = P.list ? (int64_t)*P.list_len : P.n_eval;
+    if (lb >= n_work) return;
+    const int64_t e = P.list ? (int64_t)P.list[lb] : (P.order ? (int64_t)P.order[lb] : lb);
+    if (e < 0 || e >= P.n_eval) return;
+    const int64_t s0 = P.adj_ptr ? P.adj_ptr[e] : e * (int64_t)P.k;
+    const int kk = P.adj_ptr ? (int)(P.adj_ptr[e + 1] - s0) : P.k;
+    const int64_t o0 = P.out_ptr ? P.out_ptr[e] : e * (int64_t)P.k;
+    const int M = stencil_n_mono(P, e, DIM);
+    if (M < 0) {
+        if (L == 0 && P.status) P.status[e] = RBFX_ST_INVALID;
+        return;
+    }
+    if (kk < 1 || M + kk > rows_cap) {
+        if (L == 0 && P.status) P.status[e] = RBFX_ST_TOO_LARGE;
+        return;
+    }
+    const int n = M + kk;
+
+    // ---- the stencil points this lane loads: q = L and q = L + 64
+    double xe[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xe[d] = P.eval[e * DIM + d];
+    int gidx[2] = {0, 0};
+    double D[2][DIM];
+    double r2max = 0.0;
+    bool ib[2] = {false, false};
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int q = L + 64 * t;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) D[t][d] = 0.0;
+        if (q < kk) {
+            gidx[t] = P.adj_idx[s0 + q];
+            double r2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                D[t][d] = P.data[(int64_t)gidx[t] * DIM + d] - xe[d];
+                r2 += D[t][d] * D[t][d];
+            }
+            r2max = fmax(r2max, r2);
+            ib[t] = P.is_boundary && P.is_boundary[gidx[t]];
+        }
+    }
+    // stencil classification (types.jl:208-223)
+    const bool hermite = P.is_boundary && (__ballot(ib[0] || ib[1]) != 0ull);
+    if (hermite && e < P.n_data && P.is_boundary[e]) {
+        const int bi = P.bnd_index[e];
+        if (P.bc_alpha[bi] == 1.0 && P.bc_beta[bi] == 0.0) {
+            // identity row (fill_dirichlet_entry!, execution.jl:297-303)
+            if (L < n_cols) P.out_val[(int64_t)L * P.out_nnz + o0] = 1.0;
+            if (L == 0) {
+                if (P.out_col) P.out_col[o0] = (int32_t)e;
+                if (P.status) P.status[e] = RBFX_ST_DIRICHLET;
+            }
+            return;
+        }
+    }
+    double h = sqrt(group_max<64>(r2max));
+    if (!(h > 0.0)) h = 1.0;
+    const double hinv = 1.0 / h;
+    unsigned long long at_eval[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int q = L + 64 * t;
+        bool eq = false;
+        if (q < kk) {
+            int bt = 0;
+            double ba = 0.0, bb = 0.0;
+            if (hermite && ib[t]) {
+                const int bi = P.bnd_index[gidx[t]];
+                ba = P.bc_alpha[bi];
+                bb = P.bc_beta[bi];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) s_bn[q][d] = P.normals[(int64_t)bi * DIM + d];
+                bt = (ba == 1.0 && bb == 0.0) ? 1 : 2;
+            }
+            eq = hermite;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                s_xi[q][d] = D[t][d] / h;
+                eq = eq && (D[t][d] == 0.0);
+            }
+            s_bt[q] = bt;
+            s_ba[q] = ba;
+            s_bb[q] = bb / h;  // beta acts through a first derivative
+        }
+        at_eval[t] = __ballot(eq);
+    }
+    int eval_local = 0;
+    if (hermite) {
+        if (!at_eval[0] && !at_eval[1]) {
+            if (L == 0 && P.status) P.status[e] = RBFX_ST_EVAL_NOT_FOUND;
+            return;
+        }
+        eval_local = at_eval[0] ? __builtin_ctzll(at_eval[0]) : 64 + __builtin_ctzll(at_eval[1]);
+    }
+    __syncthreads();
+
+    BasisDev B;
+    B.kind = P.basis_kind;
+    B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
+
+    // ---- assembly (FP64): rows i = L, L + 64; columns 0..n-1, then the RHS at n..n+RC-1
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int i = L + 64 * t;
+        if (i >= n) continue;
+        double* Ai = s_A + (int64_t)i * lda;
+        const bool is_poly = i < M;
+        const int pi = i - M;
+        double xi[DIM], bn[DIM];
+        int bt = 0;
+        double ba = 0.0, bb = 0.0;
+        if (!is_poly) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                xi[d] = s_xi[pi][d];
+                bn[d] = hermite ? s_bn[pi][d] : 0.0;
+            }
+            bt = s_bt[pi];
+            ba = s_ba[pi];
+            bb = s_bb[pi];
+        }
+        for (int j = 0; j < n; ++j) {
+            double v = 0.0;
+            if (j < M) {
+                // point row: monomial j at the own point (Neumann/Robin rows: alpha P + beta dP/dn)
+                if (!is_poly) {
+                    v = mono_op<DIM>(s_mono[j], RBFX_OP_VALUE, 0, 0, xi);
+                    if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[j], bn, xi);
+                }
+            } else {
+                const int jj = j - M;
+                double xj[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xj[d] = s_xi[jj][d];
+                const int tj = s_bt[jj];
+                if (is_poly) {
+                    // polynomial row i: monomial i at point jj
+                    v = mono_op<DIM>(s_mono[i], RBFX_OP_VALUE, 0, 0, xj);
+                    if (hermite && tj == 2) {
+                        double nj[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
+                        v = s_ba[jj] * v + s_bb[jj] * mono_normal<DIM>(s_mono[i], nj, xj);
+                    }
+                } else {
+                    double Dj[DIM], q2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        Dj[d] = xi[d] - xj[d];
+                        q2 += Dj[d] * Dj[d];
+                    }
+                    v = phi_r2(B, q2);
+                    if (hermite && (bt == 2 || tj == 2)) {
+                        double nj[DIM];
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
+                        double Di, Djn, D2;
+                        phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
+                        const double aj = s_ba[jj], bj = s_bb[jj];
+                        if (bt != 2) v = aj * v - bj * Djn;
+                        else if (tj != 2) v = ba * v + bb * Di;
+                        else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
+                    }
+                }
+            }
+            Ai[j] = v;
+        }
+        // right-hand sides
+        double bcol[RC];
+#pragma unroll
+        for (int c = 0; c < RC; ++c) bcol[c] = 0.0;
+        if (!is_poly) {
+            double De[DIM], q2 = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) {
+                De[d] = -xi[d];
+                q2 += De[d] * De[d];
+            }
+            for (int tt = 0; tt < n_terms; ++tt) {
+                const TermDev tm = s_terms[tt];
+                double v = phi_term<DIM>(B, tm.kind, tm.d1, tm.d2, De, q2);
+                if (hermite && bt == 2) v = ba * v + bb * phi_term_normal<DIM>(B, tm.kind, tm.d1, De, q2, bn);
+                const double sc = term_scale(tm, hinv);
+#pragma unroll
+                for (int c = 0; c < RC; ++c)
+                    if (tm.col == c) bcol[c] += sc * v;
+            }
+        } else {
+            double z[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) z[d] = 0.0;
+            const int(&ea)[3] = s_mono[i];
+            if (hermite && s_bt[eval_local] == 2) {
+                // the eval node's boundary operator applied to P (assembly.jl:452-467)
+                const double eav = s_ba[eval_local], ebv = s_bb[eval_local];
+                double en[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) en[d] = s_bn[eval_local][d];
+                const double v = eav * mono_op<DIM>(ea, RBFX_OP_VALUE, 0, 0, z) + ebv * mono_normal<DIM>(ea, en, z);
+#pragma unroll
+                for (int c = 0; c < RC; ++c) bcol[c] = v;
+            } else {
+                for (int tt = 0; tt < n_terms; ++tt) {
+                    const TermDev tm = s_terms[tt];
+                    const double v = term_scale(tm, hinv) * mono_op<DIM>(ea, tm.kind, tm.d1, tm.d2, z);
+#pragma unroll
+                    for (int c = 0; c < RC; ++c)
+                        if (tm.col == c) bcol[c] += v;
+                }
+            }
+        }
+#pragma unroll
+        for (int c = 0; c < RC; ++c) Ai[n + c] = bcol[c];
+    }
+    __syncthreads();
+
+    // ---- factorisation with rook pivoting; rows never move, pivoted rows freeze
+    const double alpha = 0.6403882032022076;  // (1 + sqrt(17)) / 8
+    const int ncol = n + RC;
+    bool act[2] = {L < n, L + 64 < n};
+    int myslot[2] = {1 << 20, 1 << 20};
+    int slot = 0;
+    bool failed = false;
+    auto A = [&](int i, int j) -> double { return s_A[(int64_t)i * lda + j]; };
+    // largest |A(r, c)| over the active rows r != excl, as a packed (magnitude, row) key
+    auto colkey = [&](int c, int excl) -> uint32_t {
+        uint32_t k = 0u;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int r = L + 64 * t;
+            if (act[t] && r != excl) k = max(k, row_key(fabs(A(r, c)), r));
+        }
+        return group_max_u32<64>(k);
+    };
+    for (;;) {
+        const unsigned long long b0 = __ballot(act[0]), b1 = __ballot(act[1]);
+        if (!b0 && !b1) break;
+        const int c0 = b0 ? (int)__builtin_ctzll(b0) : 64 + (int)__builtin_ctzll(b1);
+        // column c0 = row c0 by symmetry of the active block
+        const uint32_t ck = colkey(c0, c0);
+        int imax = ck ? 127 - (int)(ck & 127u) : c0;
+        double colmax = ck ? fabs(A(imax, c0)) : 0.0;
+        const double absakk = fabs(A(c0, c0));
+        int kstep = 1, p = c0, kp = c0;
+        bool need_rook = absakk < alpha * colmax;
+        if (!(absakk > 0.0 || colmax > 0.0)) {
+            failed = true;  // zero column: singular
+            break;
+        }
+        for (int it = 0; it < 2 * kWideRows && need_rook; ++it) {
+            const uint32_t rk = colkey(imax, imax);
+            const int ri = rk ? 127 - (int)(rk & 127u) : imax;
+            const double rowmax = rk ? fabs(A(ri, imax)) : 0.0;
+            const double aii = fabs(A(imax, imax));
+            if (!(aii < alpha * rowmax)) {
+                kp = imax;
+                need_rook = false;
+            } else if (p == ri || rowmax <= colmax || it == 2 * kWideRows - 1) {
+                kp = imax;
+                kstep = 2;
+                need_rook = false;
+            } else {
+                p = imax;
+                colmax = rowmax;
+                imax = ri;
+            }
+        }
+        const int p0 = (kstep == 1) ? kp : p;
+        const int p1 = kp;  // equals p0 for a 1x1 pivot
+        const double d0 = A(p0, p0), d1 = A(p1, p1), bq = A(p0, p1);
+        const double det = d0 * d1 - bq * bq;
+        if ((kstep == 1 && d0 == 0.0) || (kstep == 2 && det == 0.0)) {
+            failed = true;
+            break;
+        }
+        const double idet = 1.0 / det;
+        if (L == 0) {
+            s_piv[slot] = p0;
+            s_blk[slot] = kstep;
+            if (kstep == 2) {
+                s_piv[slot + 1] = p1;
+                s_blk[slot + 1] = 0;
+            }
+        }
+        const double* U0 = s_A + (int64_t)p0 * lda;
+        const double* U1 = s_A + (int64_t)p1 * lda;
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int r = L + 64 * t;
+            if (!act[t] || r == p0 || r == p1) continue;
+            const double u0 = U0[r];
+            double l0, l1 = 0.0;
+            if (kstep == 1) {
+                l0 = u0 / d0;
+            } else {
+                const double u1 = U1[r];
+                l0 = (u0 * d1 - u1 * bq) * idet;
+                l1 = (u1 * d0 - u0 * bq) * idet;
+            }
+            const double ml0 = -l0, ml1 = -l1;
+            double* Ar = s_A + (int64_t)r * lda;
+            for (int j = 0; j < ncol; ++j) Ar[j] = fma(ml0, U0[j], Ar[j]);
+            if (kstep == 2)
+                for (int j = 0; j < ncol; ++j) Ar[j] = fma(ml1, U1[j], Ar[j]);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int r = L + 64 * t;
+            if (act[t] && r == p0) {
+                act[t] = false;
+                myslot[t] = slot;
+            }
+            if (act[t] && kstep == 2 && r == p1) {
+                act[t] = false;
+                myslot[t] = slot + 1;
+            }
+        }
+        slot += kstep;
+        __syncthreads();
+    }
+    __syncthreads();
+
+    // ---- back substitution over the frozen rows, last pivot first
+    double x[2][RC], acc[2][RC];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            x[t][c] = 0.0;
+            acc[t][c] = 0.0;
+        }
+    if (!failed) {
+        const int nslots = slot;
+        for (int s = nslots - 1; s >= 0; --s) {
+            const int blk = s_blk[s];
+            const int bprev = s > 0 ? s_blk[s - 1] : -1;
+            const bool one = blk == 1, two = (blk == 0) && (bprev == 2);
+            if (!one && !two) continue;  // first slot of a 2x2 block: solved with the second
+            const int p = one ? s_piv[s] : s_piv[s - 1];
+            const int q = two ? s_piv[s] : p;
+            const int pl = p & 63, pt = p >> 6, ql = q & 63, qt = q >> 6;
+            const double dp = A(p, p), dq = A(q, q), bq = A(p, q);
+            const double idet = 1.0 / (dp * dq - bq * bq);
+            const double dinv = 1.0 / dp;
+#pragma unroll
+            for (int c = 0; c < RC; ++c) {
+                if (c >= n_cols) continue;
+                // y - acc of rows p and q, from their owning lanes
+                const double rp0 = (L < n) ? A(L, n + c) - acc[0][c] : 0.0;
+                const double rp1 = (L + 64 < n) ? A(L + 64, n + c) - acc[1][c] : 0.0;
+                const double yp = pt ? __shfl(rp1, pl, 64) : __shfl(rp0, pl, 64);
+                const double yq = qt ? __shfl(rp1, ql, 64) : __shfl(rp0, ql, 64);
+                const double x0 = two ? (yp * dq - yq * bq) * idet : yp * dinv;
+                const double x1 = two ? (yq * dp - yp * bq) * idet : 0.0;
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int r = L + 64 * t;
+                    if (r == p) x[t][c] = x0;
+                    if (two && r == q) x[t][c] = x1;
+                    const bool upd = r < n && (one ? (myslot[t] < s) : (myslot[t] < s - 1));
+                    if (upd) acc[t][c] += two ? A(r, p) * x0 + A(r, q) * x1 : A(r, p) * x0;
+                }
+            }
+        }
+    }
+
+    // ---- outputs: weights of the point rows, multipliers of the polynomial rows
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int i = L + 64 * t;
+        if (i >= n) continue;
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            if (c >= n_cols) continue;
+            const double v = failed ? __builtin_nan("") : x[t][c];
+            if (i >= M) P.out_val[(int64_t)c * P.out_nnz + o0 + (i - M)] = v;
+            else if (P.out_lambda) P.out_lambda[((int64_t)c * P.n_eval + e) * P.n_mono + i] = v;
+        }
+        // row i holds stencil point i - M (the points were loaded by lane q % 64)
+        if (i >= M && P.out_col) P.out_col[o0 + (i - M)] = P.adj_idx[s0 + (i - M)];
+    }
+    if (L == 0 && P.status) P.status[e] = failed ? RBFX_ST_SINGULAR : P.ok_status;
+}
+
+static int rc_class(int n_cols) { return n_cols <= 1 ? 1 : (n_cols <= 4 ? 4 : 8); }
+static int wide_lda(int rows_cap, int n_cols) { return (rows_cap + rc_class(n_cols)) | 1; }
+
+// the system plus the kernel's static tables within 64 KB of LDS
+constexpr int64_t kWideStaticLds = 8 * 1024;
+bool wide_fits(int rows_cap, int n_cols)
+{
+    return rows_cap >= 1 && rows_cap <= RBFX_MAX_N &&
+           (int64_t)rows_cap * wide_lda(rows_cap, n_cols) * 8 + kWideStaticLds <= 65536;
+}
+
+template <int DIM, int RC>
+static hipError_t launch_wide_rc(const BuildDev& P, int rows_cap, int64_t n_work, hipStream_t s)
+{
+    BuildDev Q = P;
+    Q.n_blocks = n_work;
+    const int lda = wide_lda(rows_cap, P.n_cols);
+    const size_t lds = (size_t)rows_cap * lda * sizeof(double);
+    hipLaunchKernelGGL((k_build_wide<DIM, RC>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q, rows_cap,
+                       lda);
+    return hipGetLastError();
+}
+
+template <int DIM>
+static hipError_t launch_wide_dim(const BuildDev& P, int rows_cap, int64_t n_work, hipStream_t s)
+{
+    switch (rc_class(P.n_cols)) {
+    case 1: return launch_wide_rc<DIM, 1>(P, rows_cap, n_work, s);
+    case 4: return launch_wide_rc<DIM, 4>(P, rows_cap, n_work, s);
+    default: return launch_wide_rc<DIM, 8>(P, rows_cap, n_work, s);
+    }
+}
+
+hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n_work, hipStream_t s)
+{
+    if (n_work <= 0) return hipSuccess;
+    if (!wide_fits(rows_cap, P.n_cols)) return hipErrorInvalidValue;
+    switch (dim) {
+    case 1: return launch_wide_dim<1>(P, rows_cap, n_work, s);
+    case 2: return launch_wide_dim<2>(P, rows_cap, n_work, s);
+    case 3: return launch_wide_dim<3>(P, rows_cap, n_work, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace rbfx

@@ -35,7 +35,7 @@ struct LearnedDev {
     int basis_kind;
     double eps;
     int n_mono;
-    int mono_e[RBFX_MAX_N][3];
+    int mono_e[kMaxMono][3];
     TermDev terms[RBFX_MAX_TERMS];
     int n_terms;
     double op_scale_order;  // derivative order of the operator (weights scale as h^-order)

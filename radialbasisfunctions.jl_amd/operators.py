@@ -380,7 +380,10 @@ class RadialBasisOperator:
     SPMV_MAPPING = 0  # rbfx_spmv_args.mapping: 0 automatic, 1 lanes per row, 2 staged through LDS
 
     def __init__(self, L, data, eval_points=None, basis=None, k=None, adjl=None, hermite=None, device=None,
-                 precision="fp64", refine_steps=4, accept_tol=1e-12, model=None, _weights=None):
+                 precision="fp64", refine_steps=4, accept_tol=1e-12, model=None, poly_deg_map=None, _weights=None):
+        """poly_deg_map (no reference counterpart): optional per-eval-point polynomial
+        degree in -1..basis.poly_deg, e.g. a higher degree for stencils near a boundary;
+        one build then mixes system sizes, which the library runs in size buckets."""
         if not isinstance(L, AbstractOperator):
             raise ArgumentError(f"{L!r} is not an operator")
         self.L = L
@@ -403,6 +406,15 @@ class RadialBasisOperator:
         self._d_data = _to_device(self.data, self.dev)
         self._d_eval = self._d_data if eval_points is None else _to_device(self.eval_points, self.dev)
         t = _lib.torch()
+        self._d_deg = None
+        if poly_deg_map is not None:
+            dm = np.asarray(poly_deg_map.cpu() if isinstance(poly_deg_map, t.Tensor) else poly_deg_map).ravel()
+            if dm.shape[0] != self.eval_points.shape[0]:
+                raise DimensionMismatch(f"poly_deg_map has {dm.shape[0]} entries for {self.eval_points.shape[0]} "
+                                        "eval points")
+            if dm.size and (dm.min() < -1 or dm.max() > self.basis.poly_deg):
+                raise ArgumentError(f"poly_deg_map entries must lie in -1..{self.basis.poly_deg}")
+            self._d_deg = t.as_tensor(dm.astype(np.int8), device=self.dev)
         if adjl is None:
             self.adj_ptr = None
             self.adj_idx = _lib.knn(self._d_data, self._d_eval, self.k)
@@ -515,6 +527,7 @@ class RadialBasisOperator:
             # depend on the order
             self._row_order = _lib.spatial_order(self._d_eval)
         a.order = _lib.ptr(self._row_order)
+        a.poly_deg_map = _lib.ptr(self._d_deg)
         self._args, self._keep = a, keep
         rc, counts = _lib.build_weights(a, self.dev)
         self.status_counts = counts

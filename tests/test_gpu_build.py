@@ -585,6 +585,161 @@ def test_c3_shape_disk_hermite_variable_k(RBF, O, gpu):
     assert st[RBF._abi.ST_REFINED] + st[RBF._abi.ST_FALLBACK64] == len(allp)
 
 
+def _c3_disk(rng, n_mix=4800, n_bg=1200, nb=400):
+    """Unit disk: a seeded 8-Gaussian mixture (sigma 0.02-0.2) clipped to the disk, a
+    uniform background (stencils made only of circle nodes are singular for degree >= 2
+    in the reference as well), and nb uniformly spaced boundary nodes."""
+    cen = rng.uniform(-0.6, 0.6, (8, 2))
+    sig = rng.uniform(0.02, 0.2, 8)
+    pts = []
+    while len(pts) < n_mix:
+        j = rng.integers(8)
+        p = cen[j] + sig[j] * rng.standard_normal(2)
+        if p @ p < 0.97:
+            pts.append(p)
+    while len(pts) < n_mix + n_bg:
+        p = rng.uniform(-1, 1, 2)
+        if p @ p < 0.97:
+            pts.append(p)
+    th = 2 * np.pi * np.arange(nb) / nb
+    bnd = np.stack([np.cos(th), np.sin(th)], 1)
+    allp = np.concatenate([np.array(pts), bnd])
+    isb = np.zeros(len(allp), dtype=bool)
+    isb[-nb:] = True
+    return allp, isb, bnd
+
+
+def _oracle_rows(O, allp, sel, adjl, deg, opk, herm_o):
+    """Extended-precision oracle weights and 2-norm condition numbers of the stencils
+    sel (one polynomial degree)."""
+    sel_adj = [adjl[i] for i in sel]
+    sptr = np.zeros(len(sel) + 1, dtype=np.int64)
+    np.cumsum([len(a) for a in sel_adj], out=sptr[1:])
+    sidx = np.concatenate(sel_adj).astype(np.int32)
+    _, _, wld, _, rc = O.build(allp, allp[sel], (sptr, sidx), O.basis_struct(3, deg), opk, scaled=True,
+                               long_double=True, hermite=herm_o)
+    assert rc == 0
+    kap = O.condition_numbers(allp, allp[sel], (sptr, sidx), O.basis_struct(3, deg), opk, hermite=herm_o)
+    return sptr, wld[0], np.where(np.isfinite(kap), kap, 1.0)
+
+
+def test_c3_config_mixed_degree_and_wide_stencils(RBF, O, gpu):
+    """The uneven-size configuration as specified: unit disk (Gaussian mixture + Neumann
+    boundary nodes), k adapted to local density in [12, 60], polynomial degree 3 for the
+    stencils that touch the boundary and 2 elsewhere (one build, per-stencil degree map),
+    partial dx + d2y, FP32 factorisation with FP64 refinement. n = k + m reaches 70, so
+    the build fills all three size buckets: two stencils per wavefront (n <= 32), one
+    (33..64), and the system in LDS (65..70). Weights of every degree group against the
+    extended-precision oracle at the conditioning bound; the applied operator against
+    the analytic derivatives; and each bucket run on every kernel that serves it (FP64)
+    agrees with the default choice."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(34)
+    allp, isb, bnd = _c3_disk(rng)
+    nb = len(bnd)
+    tree = cKDTree(allp)
+    d12 = tree.query(allp, 13)[0][:, -1]
+    kk = np.clip((24 * (np.median(d12) / d12) ** 2).astype(int), 12, 60)
+    kk[isb] = 60
+    adjl = [tree.query(allp[i], kk[i])[1].astype(np.int32) for i in range(len(allp))]
+    touch = np.array([isb[a].any() for a in adjl])
+    deg = np.where(touch, 3, 2).astype(np.int8)
+    herm = RBF.Hermite(isb, [RBF.Neumann()] * nb, bnd)
+    herm_o = (isb, np.zeros(nb), np.ones(nb), bnd)
+    L = RBF.Partial(1, 1) + RBF.Partial(2, 2)
+    opk = _op(O, 1, [(0, O.abi.OP_D1, 0, 0, 1.0), (0, O.abi.OP_D2, 1, 0, 1.0)])
+    nn = kk + np.where(deg == 3, 10, 6)
+    assert nn.max() == 70 and (nn <= 32).any() and ((nn > 32) & (nn <= 64)).any() and (nn > 64).any()
+
+    op = RBF.RadialBasisOperator(L, allp, basis=RBF.PHS(3, poly_deg=3), adjl=adjl, hermite=herm, device=gpu,
+                                 precision="fp32_refined", poly_deg_map=deg)
+    n_b, k_b = RBF._lib.last_build_buckets()
+    assert n_b == [int((nn <= 32).sum()), int(((nn > 32) & (nn <= 64)).sum()), int((nn > 64).sum())]
+    assert k_b[2] == 2
+    st = op.status_counts
+    assert st[RBF._abi.ST_REFINED] + st[RBF._abi.ST_FALLBACK64] == len(allp)
+    ptr = op.row_ptr.cpu().numpy()
+    vals = op.vals.cpu().numpy()[0]
+    # every row's column indices are its stencil, in stencil order (all three kernels)
+    assert np.array_equal(op.col.cpu().numpy(), np.concatenate(adjl))
+    bad = np.flatnonzero(~np.isfinite(vals))
+    assert bad.size == 0, (bad.size, np.searchsorted(ptr, bad[:10], side="right") - 1)
+    for dg in (2, 3):
+        grp = np.flatnonzero(deg == dg)
+        sel = np.concatenate([rng.choice(grp[~isb[grp]], min(150, int((~isb[grp]).sum())), replace=False),
+                              grp[isb[grp]][::8]])
+        sptr, wld, kap = _oracle_rows(O, allp, sel, adjl, dg, opk, herm_o)
+        for j, i in enumerate(sel):
+            g = vals[ptr[i]:ptr[i + 1]]
+            t = wld[sptr[j]:sptr[j + 1]]
+            eg = np.abs(g - t).max() / np.abs(t).max()
+            assert eg <= max(1e-10, kap[j] * U64), (dg, i, nn[i], eg, kap[j] * U64)
+
+    # applied operator: dx f + d2y f of sin4x cos3y away from the boundary
+    x, y = allp[:, 0], allp[:, 1]
+    f = np.sin(4 * x) * np.cos(3 * y)
+    ex = 4 * np.cos(4 * x) * np.cos(3 * y) - 9 * np.sin(4 * x) * np.cos(3 * y)
+    inner = (x ** 2 + y ** 2 < 0.6 ** 2)
+    out = op(f)
+    bad = np.flatnonzero(~np.isfinite(out))
+    assert bad.size == 0, (bad[:10], [(ptr[i + 1] - ptr[i], nn[i]) for i in bad[:10]],
+                           np.isfinite(op.vals.cpu().numpy()[0]).all(), op._max_row)
+    # NRMSE (measured 0.045): single rows of the sparse background (k = 12) carry O(1)
+    # discretisation error, the same with the oracle's weights (pinned above)
+    e = np.sqrt(np.mean((out - ex)[inner] ** 2) / np.mean(ex[inner] ** 2))
+    assert e < 0.08, e
+
+    # every kernel that serves a bucket gives the weights of the default choice (FP64)
+    base = RBF.RadialBasisOperator(L, allp, basis=RBF.PHS(3, poly_deg=3), adjl=adjl, hermite=herm, device=gpu,
+                                   poly_deg_map=deg)
+    wb = base.vals.cpu().numpy()[0]
+    for bucket, kern in ((0, 1), (1, 2)):
+        RBF._lib.tune_bucket(bucket, kern)
+        try:
+            alt = RBF.RadialBasisOperator(L, allp, basis=RBF.PHS(3, poly_deg=3), adjl=adjl, hermite=herm,
+                                          device=gpu, poly_deg_map=deg)
+            assert RBF._lib.last_build_buckets()[1][bucket] == kern
+        finally:
+            RBF._lib.tune_bucket(bucket, -1)
+        wa = alt.vals.cpu().numpy()[0]
+        in_b = np.flatnonzero((nn <= 32) if bucket == 0 else ((nn > 32) & (nn <= 64)))[::5]
+        for dg in (2, 3):
+            sel = in_b[deg[in_b] == dg]
+            if len(sel) == 0:
+                continue
+            _, _, kap = _oracle_rows(O, allp, sel, adjl, dg, opk, herm_o)
+            for j, i in enumerate(sel):
+                a_, b_ = wa[ptr[i]:ptr[i + 1]], wb[ptr[i]:ptr[i + 1]]
+                d = np.abs(a_ - b_).max() / np.abs(b_).max()
+                # two FP64 orderings of one solve, each within kappa u of the truth
+                assert d <= max(1e-10, 2 * kap[j] * U64), (bucket, i, d, kap[j] * U64)
+
+
+def test_degree_map_validation(RBF, O, gpu):
+    """A degree map entry outside -1..poly_deg, or of the wrong length, is refused."""
+    rng = np.random.default_rng(3)
+    pts = rng.random((500, 2))
+    with pytest.raises(RBF.ArgumentError):
+        RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu, poly_deg_map=np.full(500, 3))
+    with pytest.raises(RBF.DimensionMismatch):
+        RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu, poly_deg_map=np.full(499, 2))
+    # degrees 0 and 1 mixed with the basis degree, against the oracle
+    deg = rng.integers(0, 3, 500).astype(np.int8)
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu, poly_deg_map=deg)
+    adj = op.adj_idx.cpu().numpy()
+    vals = op.vals.cpu().numpy()[0].reshape(500, 20)
+    lap = O.abi.make_operator(1, [(0, O.abi.OP_LAP, 0, 0, 1.0)])
+    for dg in (0, 1, 2):
+        sel = np.flatnonzero(deg == dg)
+        _, _, w, _, rc = O.build(pts, pts[sel], adj[sel], O.basis_struct(3, dg), lap, scaled=True,
+                                 long_double=True)
+        assert rc == 0
+        w = w[0].reshape(-1, 20)
+        err = np.abs(vals[sel] - w).max(1) / np.abs(w).max(1)
+        assert np.quantile(err, 0.99) <= 1e-9, (dg, np.quantile(err, 0.99))
+
+
 def test_c5_shape_regrid_imq_multifield(RBF, O, gpu):
     """3-D regrid: uniform sources to a jittered target grid, IMQ eps=3, degree 1,
     k=30, FP32 build + FP64 refinement, 16 fields applied at once; the applied values
