@@ -138,6 +138,182 @@ def time_apply(op, X, stream, reps=20):
     return ev0.elapsed_time(ev1) / reps
 
 
+# ------------------------------------------------------------------ other configurations
+def _build_ms(R, op, reps=2):
+    """Device time of whole builds (HIP events the library records on its stream)."""
+    ts = []
+    for _ in range(reps):
+        op.update_weights()
+        ts.append(R._lib.last_build_timing()[3])
+    return float(np.median(ts))
+
+
+def _status(R, op):
+    sc = op.status_counts
+    return {"ok": sc[0], "dirichlet": sc[1], "refined": sc[2], "fallback64": sc[3]}
+
+
+def config_c2(R, dev):
+    """3-D gradient + Laplacian: 126^3 jittered grid (2,000,376 points), PHS(5),
+    degree 3, k = 40, one fused build with 4 RHS columns (n = 60, r = 4)."""
+    import torch
+
+    m = 126
+    h = 1.0 / (m - 1)
+    lin = np.linspace(0.0, 1.0, m)
+    g = np.stack(np.meshgrid(lin, lin, lin, indexing="ij"), -1).reshape(-1, 3)
+    pts = g + np.random.default_rng(20260102).uniform(-0.3 * h, 0.3 * h, g.shape)
+    d = torch.as_tensor(pts, device=dev)
+    adj = R._lib.knn(d, d, 40)
+    L = R.Stacked(R.Jacobian(3), R.Laplacian())
+    out = {"workload": "gradient+laplacian-3d-2M-phs5-deg3-k40 (jittered 126^3 grid; 4 columns, one factorisation)",
+           "n_points": len(pts)}
+    flop = 60 ** 3 / 3 + 2 * 60 * 60 * 4
+    for prec in ("fp64", "fp32_refined"):
+        op = R.RadialBasisOperator(L, d, basis=R.PHS(5, poly_deg=3), k=40, adjl=adj, precision=prec)
+        ms = _build_ms(R, op)
+        tf = flop * len(pts) / (ms * 1e-3) / 1e12
+        out[prec] = {"build_ms": ms, "stencils_per_s": len(pts) / (ms * 1e-3), "tflops": tf,
+                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op)}
+        if prec == "fp64":
+            x, y, z = d.unbind(1)
+            f = torch.exp(x) * torch.sin(2 * y) * torch.cos(3 * z)
+            ex = torch.stack([f, 2 * torch.exp(x) * torch.cos(2 * y) * torch.cos(3 * z),
+                              -3 * torch.exp(x) * torch.sin(2 * y) * torch.sin(3 * z), -12 * f], 1)
+            G = op(f)
+            inner = ((d > 0.1) & (d < 0.9)).all(1)
+            e = ((G - ex)[inner] ** 2).mean(0).sqrt() / (ex[inner] ** 2).mean(0).sqrt()
+            out["nrmse_dx_dy_dz_lap"] = [float(v) for v in e.cpu()]
+            del op, G
+    return out
+
+
+def config_c3(R, dev):
+    """Uneven build on the unit disk: 1,000,000 points from a seeded mixture of 8
+    Gaussians (sigma 0.02-0.2) clipped to the disk, 20,000 uniformly spaced boundary
+    nodes with Neumann rows, plus a uniform background of 250,000 points and a layer of
+    20,000 points next to the boundary (stencils made mostly of circle nodes are
+    singular for degree >= 2, in the reference too);
+    k from local density in [12, 60] (60 at the boundary), degree 3 for the stencils
+    that touch the boundary and 2 elsewhere (n = 18..70), dx + d2y, FP32 factorisation
+    with FP64 refinement (and plain FP64 beside it)."""
+    import torch
+
+    rng = np.random.default_rng(20260103)
+    cen = rng.uniform(-0.6, 0.6, (8, 2))
+    sig = rng.uniform(0.02, 0.2, 8)
+    parts, got = [], 0
+    while got < 1_000_000:
+        j = rng.integers(8, size=1_500_000)
+        p = cen[j] + sig[j, None] * rng.standard_normal((len(j), 2))
+        p = p[(p ** 2).sum(1) < 0.995]
+        parts.append(p)
+        got += len(p)
+    mix = np.concatenate(parts)[:1_000_000]
+    bg = rng.uniform(-1, 1, (400_000, 2))
+    bg = bg[(bg ** 2).sum(1) < 0.995][:250_000]
+    # a near-boundary layer (0.97 < r < 0.995): with k = 60 a boundary stencil is mostly
+    # circle nodes, on which x^2 + y^2 - 1 (times 1, x, y) vanishes; interior points in
+    # the stencil restore the rank of the degree-3 block
+    rr = np.sqrt(rng.uniform(0.97 ** 2, 0.995 ** 2, 20_000))
+    tt = rng.uniform(0, 2 * np.pi, 20_000)
+    layer = np.stack([rr * np.cos(tt), rr * np.sin(tt)], 1)
+    nb = 20_000
+    th = 2 * np.pi * np.arange(nb) / nb
+    bnd = np.stack([np.cos(th), np.sin(th)], 1)
+    pts = np.concatenate([mix, bg, layer, bnd])
+    n = len(pts)
+    isb = np.zeros(n, dtype=bool)
+    isb[-nb:] = True
+    d = torch.as_tensor(pts, device=dev)
+    d12 = (d - d[R._lib.knn(d, d, 13)[:, -1].long()]).norm(dim=1)
+    kk = torch.clamp((24 * (d12.median() / d12) ** 2).long(), 12, 60)
+    kk[torch.as_tensor(isb, device=dev)] = 60
+    ptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    ptr[1:] = torch.cumsum(kk, 0)
+    idx = R._lib.knn(d, d, 60, q_ptr=ptr)
+    isb_d = torch.as_tensor(isb, device=dev)
+    hits = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    hits[1:] = torch.cumsum(isb_d[idx.long()].long(), 0)[ptr[1:] - 1]
+    touch = (hits[1:] - hits[:-1]) > 0
+    deg = torch.where(touch, 3, 2).to(torch.int8)
+    herm = R.Hermite(isb, [R.Neumann()] * nb, bnd)
+    Lop = R.Partial(1, 1) + R.Partial(2, 2)
+    nn = (kk + torch.where(touch, 10, 6)).double()
+    flop = float((nn ** 3 / 3 + 2 * nn ** 2).sum())
+    out = {"workload": "dx+d2y-2d-disk-1.29M-phs3-deg2/3-k12..60-hermite-neumann", "n_points": n,
+           "n_boundary": nb, "k_range": [int(kk.min()), int(kk.max())], "n_range": [int(nn.min()), int(nn.max())]}
+    for prec in ("fp32_refined", "fp64"):
+        op = R.RadialBasisOperator(Lop, d, basis=R.PHS(3, poly_deg=3), adjl=(ptr, idx), hermite=herm,
+                                   precision=prec, poly_deg_map=deg)
+        ms = _build_ms(R, op)
+        nbk, kern = R._lib.last_build_buckets()
+        tf = flop / (ms * 1e-3) / 1e12
+        out[prec] = {"build_ms": ms, "stencils_per_s": n / (ms * 1e-3), "tflops": tf,
+                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op),
+                     "buckets_n<=32,33..64,65..": nbk,
+                     "bucket_kernels": [["regs x2", "regs", "lds", "-"][k] for k in kern]}
+        if prec == "fp32_refined":
+            x, y = d.unbind(1)
+            f = torch.sin(4 * x) * torch.cos(3 * y)
+            ex = 4 * torch.cos(4 * x) * torch.cos(3 * y) - 9 * torch.sin(4 * x) * torch.cos(3 * y)
+            inner = (x ** 2 + y ** 2) < 0.36
+            v = op(f)
+            out["nrmse_inner"] = float(((v - ex)[inner] ** 2).mean().sqrt() / (ex[inner] ** 2).mean().sqrt())
+        del op
+    return out
+
+
+def config_c5(R, dev, stream):
+    """3-D regrid: 4M sources uniform in [0,1]^3 to 8M targets on a jittered 200^3 grid,
+    IMQ eps = 3, degree 1, k = 30, FP32 build + FP64 refinement; 16 fields (8 Fourier
+    modes each, wavenumber <= 6) applied at once."""
+    import torch
+
+    rng = np.random.default_rng(20260104)
+    src = rng.random((4_000_000, 3))
+    m = 200
+    h = 1.0 / (m - 1)
+    lin = np.linspace(0.0, 1.0, m)
+    tg = np.stack(np.meshgrid(lin, lin, lin, indexing="ij"), -1).reshape(-1, 3)
+    tg = np.clip(tg + rng.uniform(-0.3 * h, 0.3 * h, tg.shape), 0, 1)
+    ds = torch.as_tensor(src, device=dev)
+    dt = torch.as_tensor(tg, device=dev)
+    del src, tg
+    op = R.regrid(ds, dt, basis=R.IMQ(3.0, poly_deg=1), k=30, device=dev, precision="fp32_refined")
+    ms = _build_ms(R, op)
+    n_t = dt.shape[0]
+    flop = 34 ** 3 / 3 + 2 * 34 ** 2
+    tf = flop * n_t / (ms * 1e-3) / 1e12
+    out = {"workload": "regrid-3d-4M-to-8M-imq3-deg1-k30-16fields", "n_src": int(ds.shape[0]), "n_tgt": n_t,
+           "fp32_refined": {"build_ms": ms, "stencils_per_s": n_t / (ms * 1e-3), "tflops": tf,
+                            "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op)}}
+    kap = []
+    while len(kap) < 16 * 8:
+        v = rng.integers(-6, 7, 3)
+        if 0 < np.sqrt((v ** 2).sum()) <= 6:
+            kap.append(v)
+    kap = torch.as_tensor(np.array(kap, dtype=np.float64).reshape(16, 8, 3), device=dev)
+    ph = torch.as_tensor(rng.uniform(0, 2 * np.pi, (16, 8)), device=dev)
+    amp = torch.as_tensor(rng.standard_normal((16, 8)), device=dev)
+
+    def fields(x):
+        return torch.stack([(amp[f] * torch.cos(2 * np.pi * (x @ kap[f].T) + ph[f])).sum(1) for f in range(16)], 1)
+
+    X = fields(ds).contiguous()
+    T = fields(dt)
+    Y = op._spmv(X)[0]
+    err = float(((Y - T) ** 2).mean().sqrt() / (T ** 2).mean().sqrt())
+    apply_ms = time_apply(op, X, stream, reps=10)
+    nnz = n_t * 30
+    by = nnz * (4 + 8) + 8 * 16 * (ds.shape[0] + n_t)
+    gbs = by / (apply_ms * 1e-3) / 1e9
+    out["apply"] = {"ms": apply_ms, "GB_s": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS, "bytes": by,
+                    "kernel": "k_spmv_fields (one lane per row and field, rows in Morton order)"}
+    out["interp_nrmse"] = err
+    return out
+
+
 def main():
     if len(sys.argv) == 3 and sys.argv[1] == "--summarize":
         return summarize(sys.argv[2])
@@ -151,6 +327,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--model", default=str(MODEL_PATH))
+    ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (C2, C3, C5)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced "
@@ -279,6 +456,19 @@ def main():
                "sample": f"{ns} stencils (the first {ns} eval points of the same 4M cloud), "
                          f"C restatement of the reference arithmetic (rook LDL^T per stencil), OpenMP"}
 
+    configs = None
+    if not args.no_configs:
+        del ops, op
+        torch.cuda.empty_cache()
+        configs = {}
+        for name, fn in (("c2", lambda: config_c2(R, dev)), ("c3", lambda: config_c3(R, dev)),
+                         ("c5", lambda: config_c5(R, dev, stream))):
+            t0 = time.time()
+            configs[name] = fn()
+            configs[name]["wall_s"] = time.time() - t0
+            torch.cuda.empty_cache()
+            print(f"config {name} done in {time.time() - t0:.1f} s", file=sys.stderr, flush=True)
+
     ratio = lr["nrmse"] / plain["nrmse"]
     out = {
         "metric": "weight-build stencils/s, learned+FP32-refined vs plain FP64, at equal operator err",
@@ -318,6 +508,7 @@ def main():
                                 "traffic": profiled_traffic("k_build<double,32,2,1>"),
                                 "per_unit_flops": flops_per_stencil(n_sys, 1), "kernel_ms": stage_ms["fp64"][0]},
         "cpu_baseline": cpu,
+        "configs": configs,
     }
     print(json.dumps(out))
 

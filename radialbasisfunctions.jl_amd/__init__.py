@@ -12,7 +12,7 @@ from .basis import (PHS, PHS1, PHS3, PHS5, PHS7, IMQ, Gaussian, MonomialBasis, A
                     is_internal, check_poly_deg, n_monomials)
 from .errors import ArgumentError, DimensionMismatch, SingularException
 from .operators import (AbstractOperator, RadialBasisOperator, Laplacian, Partial, MixedPartial, Jacobian, Hessian,
-                        Regrid, Identity, ScaledOperator, SumOperator, Custom, Hermite, laplacian, partial,
+                        Regrid, Identity, ScaledOperator, SumOperator, Custom, Stacked, Hermite, laplacian, partial,
                         mixed_partial, jacobian, gradient, hessian, regrid, custom, find_neighbors, autoselect_k,
                         reorder_points,
                         weights, update_weights, is_cache_valid, print_op)

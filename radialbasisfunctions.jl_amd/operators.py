@@ -212,6 +212,34 @@ class SumOperator(AbstractOperator):
         return " + ".join(print_op(o) for o in self.ops)
 
 
+class Stacked(AbstractOperator):
+    """Several operators sharing one stencil set, built as one system with one RHS
+    column per output (no reference counterpart: the reference builds `gradient` and
+    `laplacian` as two operators, i.e. two factorisations per stencil and two index
+    arrays). Rank 1: applying it gives one output column per stacked column, in order
+    (a Jacobian contributes `dim` columns)."""
+
+    rank = 1
+
+    def __init__(self, *ops):
+        if not ops or not all(isinstance(o, AbstractOperator) for o in ops):
+            raise ArgumentError("Stacked takes one or more operators")
+        if any(o.rank > 1 for o in ops):
+            raise ArgumentError("Stacked takes operators of rank 0 or 1")
+        self.ops = ops
+
+    def lower(self, dim):
+        n, terms = 0, []
+        for o in self.ops:
+            nc, t = o.lower(dim)
+            terms += [(c + n, kind, d1, d2, coeff) for c, kind, d1, d2, coeff in t]
+            n += nc
+        return n, terms
+
+    def __repr__(self):
+        return "Stacked(" + ", ".join(repr(o) for o in self.ops) + ")"
+
+
 class Custom(AbstractOperator):
     """A custom operator given as an operator-algebra tree.
 
@@ -239,8 +267,8 @@ def print_op(op):
 # ============================================================== helpers
 def autoselect_k(data, basis):
     """min(N, max(2·binomial(m+d, d), 2d+1)) (utils.jl:39-43)."""
-    data = np.asarray(data)
-    N, d = data.shape
+    shape = tuple(data.shape) if hasattr(data, "shape") else np.asarray(data).shape
+    N, d = shape[0], (shape[1] if len(shape) > 1 else 1)
     m = basis.poly_deg
     return min(N, max(2 * n_monomials(d, m) if m >= 0 else 0, 2 * d + 1))
 
@@ -353,7 +381,19 @@ def _hermite_arrays(hermite, n_data, dim, dev):
 
 
 def _adjacency(adjl, n_eval):
-    """Accept an (N_eval, k) array or a list of index vectors (ragged -> CSR)."""
+    """Accept an (N_eval, k) array, a list of index vectors (ragged -> CSR), or a CSR
+    pair (ptr, idx) of numpy arrays or device tensors."""
+    if isinstance(adjl, tuple) and len(adjl) == 2:
+        t = _lib.torch()
+        ptr, idx = adjl
+        ptr_h = ptr.cpu().numpy() if isinstance(ptr, t.Tensor) else np.asarray(ptr)
+        if ptr_h.shape[0] != n_eval + 1:
+            raise DimensionMismatch(f"adjacency has {ptr_h.shape[0] - 1} stencils for {n_eval} eval points")
+        lens = np.diff(ptr_h)
+        idx = idx.to(t.int32) if isinstance(idx, t.Tensor) else np.ascontiguousarray(idx, dtype=np.int32)
+        if lens.size and np.all(lens == lens[0]):
+            return None, idx.reshape(n_eval, int(lens[0])), int(lens[0])
+        return ptr_h.astype(np.int64), idx, int(lens.max()) if lens.size else 0
     if isinstance(adjl, np.ndarray) and adjl.ndim == 2:
         return None, np.ascontiguousarray(adjl, dtype=np.int32), adjl.shape[1]
     t = _lib.torch()

@@ -716,6 +716,41 @@ def test_c3_config_mixed_degree_and_wide_stencils(RBF, O, gpu):
                 assert d <= max(1e-10, 2 * kap[j] * U64), (bucket, i, d, kap[j] * U64)
 
 
+def test_stacked_operator_and_csr_adjacency(RBF, gpu):
+    """Stacked(Jacobian, Laplacian) builds one system with d + 1 RHS columns: its
+    columns equal the separate operators' weights and its apply returns one output
+    column per stacked column. A CSR (ptr, idx) adjacency gives the same operator as
+    the list of index vectors it encodes."""
+    import torch
+
+    rng = np.random.default_rng(12)
+    pts = rng.random((4000, 2))
+    st = RBF.RadialBasisOperator(RBF.Stacked(RBF.Jacobian(2), RBF.Laplacian()), pts,
+                                 basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
+    jac = RBF.RadialBasisOperator(RBF.Jacobian(2), pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
+    lap = RBF.RadialBasisOperator(RBF.Laplacian(), pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
+    ws = st.vals.cpu().numpy()
+    wsep = np.concatenate([jac.vals.cpu().numpy(), lap.vals.cpu().numpy()], 0)
+    assert ws.shape == (3, 4000 * 20)
+    assert np.abs(ws - wsep).max() <= 1e-12 * np.abs(wsep).max()
+    f = np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1])
+    out = st(f)
+    assert out.shape == (4000, 3)
+    assert np.allclose(out[:, :2], jac(f), rtol=0, atol=1e-10 * np.abs(jac(f)).max())
+    assert np.allclose(out[:, 2], lap(f), rtol=0, atol=1e-10 * np.abs(lap(f)).max())
+
+    ks = rng.integers(12, 31, 4000)
+    full = RBF.find_neighbors(pts, k=30, device=gpu)
+    adjl = [full[i, :ks[i]] for i in range(4000)]
+    ptr = np.zeros(4001, dtype=np.int64)
+    np.cumsum(ks, out=ptr[1:])
+    idx = np.concatenate(adjl)
+    a = RBF.laplacian(pts, adjl=adjl, device=gpu)
+    b = RBF.laplacian(pts, adjl=(torch.as_tensor(ptr, device=gpu), torch.as_tensor(idx, device=gpu)), device=gpu)
+    assert np.array_equal(a.row_ptr.cpu().numpy(), b.row_ptr.cpu().numpy())
+    assert np.array_equal(a.vals.cpu().numpy(), b.vals.cpu().numpy())
+
+
 def test_degree_map_validation(RBF, O, gpu):
     """A degree map entry outside -1..poly_deg, or of the wrong length, is refused."""
     rng = np.random.default_rng(3)
