@@ -139,6 +139,9 @@ def time_apply(op, X, stream, reps=20):
 
 
 # ------------------------------------------------------------------ other configurations
+KERNEL_NAMES = {-1: "-", 0: "regs x2", 1: "regs", 2: "lds", 3: "regs x2 (fp64)", 4: "regs (fp64)"}
+
+
 def _build_ms(R, op, reps=2):
     """Device time of whole builds (HIP events the library records on its stream)."""
     ts = []
@@ -150,7 +153,7 @@ def _build_ms(R, op, reps=2):
 
 def _status(R, op):
     sc = op.status_counts
-    return {"ok": sc[0], "dirichlet": sc[1], "refined": sc[2], "fallback64": sc[3]}
+    return {"ok": sc[0], "dirichlet": sc[1], "refined": sc[2], "fallback64": sc[3], "fp64": sc[5]}
 
 
 def config_c2(R, dev):
@@ -174,7 +177,8 @@ def config_c2(R, dev):
         ms = _build_ms(R, op)
         tf = flop * len(pts) / (ms * 1e-3) / 1e12
         out[prec] = {"build_ms": ms, "stencils_per_s": len(pts) / (ms * 1e-3), "tflops": tf,
-                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op)}
+                     "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op),
+                     "kernel": KERNEL_NAMES[max(R._lib.last_build_buckets()[1])]}
         if prec == "fp64":
             x, y, z = d.unbind(1)
             f = torch.exp(x) * torch.sin(2 * y) * torch.cos(3 * z)
@@ -252,7 +256,7 @@ def config_c3(R, dev):
         out[prec] = {"build_ms": ms, "stencils_per_s": n / (ms * 1e-3), "tflops": tf,
                      "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op),
                      "buckets_n<=32,33..64,65..": nbk,
-                     "bucket_kernels": [["regs x2", "regs", "lds", "-"][k] for k in kern]}
+                     "bucket_kernels": [KERNEL_NAMES[k] for k in kern]}
         if prec == "fp32_refined":
             x, y = d.unbind(1)
             f = torch.sin(4 * x) * torch.cos(3 * y)
@@ -287,7 +291,8 @@ def config_c5(R, dev, stream):
     tf = flop * n_t / (ms * 1e-3) / 1e12
     out = {"workload": "regrid-3d-4M-to-8M-imq3-deg1-k30-16fields", "n_src": int(ds.shape[0]), "n_tgt": n_t,
            "fp32_refined": {"build_ms": ms, "stencils_per_s": n_t / (ms * 1e-3), "tflops": tf,
-                            "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op)}}
+                            "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op),
+                            "kernel": KERNEL_NAMES[max(R._lib.last_build_buckets()[1])]}}
     kap = []
     while len(kap) < 16 * 8:
         v = rng.integers(-6, 7, 3)
@@ -309,8 +314,22 @@ def config_c5(R, dev, stream):
     by = nnz * (4 + 8) + 8 * 16 * (ds.shape[0] + n_t)
     gbs = by / (apply_ms * 1e-3) / 1e9
     out["apply"] = {"ms": apply_ms, "GB_s": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS, "bytes": by,
-                    "kernel": "k_spmv_fields (one lane per row and field, rows in Morton order)"}
+                    "kernel": "k_spmv_fields_staged (a lane per row and field, rows in Morton order, LDS-staged)"}
     out["interp_nrmse"] = err
+    del op, Y
+    # the same regrid after reorder_points on the sources: a source's 16 field values
+    # are then gathered by neighbouring rows while its line is in L2
+    R.reorder_points(ds, 30)
+    op = R.regrid(ds, dt, basis=R.IMQ(3.0, poly_deg=1), k=30, device=dev)
+    out["fp64"] = {"build_ms": _build_ms(R, op, reps=1), "status": _status(R, op),
+                   "kernel": KERNEL_NAMES[max(R._lib.last_build_buckets()[1])], "sources": "after reorder_points"}
+    out["fp64"]["stencils_per_s"] = n_t / (out["fp64"]["build_ms"] * 1e-3)
+    X = fields(ds).contiguous()
+    q_ms = time_apply(op, X, stream, reps=10)
+    out["apply"]["reordered_sources"] = {"ms": q_ms, "GB_s": by / (q_ms * 1e-3) / 1e9,
+                                         "frac_hbm_peak": by / (q_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                         "interp_nrmse": float(((op._spmv(X)[0] - T) ** 2).mean().sqrt()
+                                                               / (T ** 2).mean().sqrt())}
     return out
 
 
@@ -328,6 +347,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--model", default=str(MODEL_PATH))
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (C2, C3, C5)")
+    ap.add_argument("--only", choices=["c2", "c3", "c5"], help="run one other configuration alone and print it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced "
@@ -342,6 +362,10 @@ def main():
     R = rbfx_loader.load()
     L = R.learned
     dev = R._lib.device_for("cuda:0")
+    if args.only:
+        fn = {"c2": config_c2, "c3": config_c3, "c5": lambda R, d: config_c5(R, d, torch.cuda.current_stream(d))}
+        print(json.dumps({args.only: fn[args.only](R, dev)}))
+        return
     k, deg = 20, 2
     m = 6
     n_sys = k + m
@@ -377,9 +401,14 @@ def main():
     results = {}
     ops = {}
     stage_ms = {}
-    for method in ("fp64", "fp32_refined", "learned"):
+    learned_kernel = None
+    # "fp32_kernel": FP32 + refinement with the FP32 register kernel pinned (the
+    # automatic choice solves this size class in FP64 when that measures faster)
+    for method in ("fp64", "fp32_refined", "learned", "fp32_kernel"):
+        prec = "fp32_refined" if method == "fp32_kernel" else method
+        R._lib.tune_bucket(0, 0 if method == "fp32_kernel" else -1)
         op = R.RadialBasisOperator(R.Laplacian(), d_pts, basis=R.PHS(3, poly_deg=deg), k=k, adjl=adj,
-                                   precision=method, model=model if method == "learned" else None)
+                                   precision=prec, model=model if method == "learned" else None)
         for _ in range(args.warmup):
             op.update_weights()
         torch.cuda.synchronize()
@@ -398,11 +427,16 @@ def main():
         y = op(d_f).cpu().numpy()
         nrmse = float(np.sqrt(np.mean((y - exact) ** 2)) / np.sqrt(np.mean(exact ** 2)))
         sc = op.status_counts
+        kern = max(R._lib.last_build_buckets()[1])
         results[method] = dict(ms_per_build=gpu_ms, wall_ms_per_build=1e3 * wall / args.steps,
                                stencils_per_s=args.n / (gpu_ms * 1e-3), nrmse=nrmse,
-                               stage_ms=stage_ms[method],
-                               status={"ok": sc[0], "refined": sc[2], "fallback64": sc[3], "learned": sc[4]})
+                               stage_ms=stage_ms[method], solve_kernel=KERNEL_NAMES[kern],
+                               status={"ok": sc[0], "refined": sc[2], "fallback64": sc[3], "learned": sc[4],
+                                       "fp64": sc[5]})
+        if method == "learned":
+            learned_kernel = kern
         ops[method] = op
+    R._lib.tune_bucket(0, -1)
 
     # apply: y = W f (Laplacian, 1 field), rows walked in spatial order (apply plan)
     op = ops["fp64"]
@@ -432,6 +466,11 @@ def main():
         dom = {"kernel": "k_learned (surrogate + FP64 acceptance)", "trace_name": "k_learned",
                "bound": "mfma", "unit": "TFLOP/s",
                "peak": BF16_PEAK_TFLOPS, "per_unit": mlp_flops(), "units": args.n, "ms": st[0]}
+    elif learned_kernel == 3:
+        dom = {"kernel": "k_build<double,32,2,1> (fallback solve; the autotuner picks FP64 over FP32 here)",
+               "trace_name": "k_build<double,32,2,1>", "bound": "valu-fp64", "unit": "TFLOP/s",
+               "peak": FP64_PEAK_TFLOPS, "per_unit": flops_per_stencil(n_sys, 1), "units": n_rej,
+               "ms": st[1]}
     else:
         dom = {"kernel": "k_build<float> (FP32 fallback solve)", "trace_name": "k_build<float,32,2,1>",
                "bound": "valu-fp32 (unpacked: FP64 rate)", "unit": "TFLOP/s",
@@ -481,7 +520,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16-mfma+f64-accept / f32+f64-refine fallback",
+        "dtype": ("bf16-mfma+f64-accept / f64 fallback (autotuned over f32+f64-refine)"
+                  if learned_kernel in (2, 3, 4) else "bf16-mfma+f64-accept / f32+f64-refine fallback"),
         "data": "synthetic (seeded 2-D cloud, density x10 across the domain)",
         "config": {"workload": "laplacian-2d-4M-phs3-deg2-k20-learned", "n_points": args.n, "k": k,
                    "poly_deg": deg, "basis": "PHS3", "method": "learned"},

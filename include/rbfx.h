@@ -85,6 +85,8 @@ enum {
     RBFX_ST_REFINED = 2,         /* FP32 factorisation + FP64 refinement accepted   */
     RBFX_ST_FALLBACK64 = 3,      /* refinement did not converge: FP64 re-solve      */
     RBFX_ST_LEARNED = 4,         /* surrogate prediction accepted                   */
+    RBFX_ST_FP64 = 5,            /* FP32_REFINED / LEARNED fallback: solved directly in
+                                    FP64, the faster kernel for its size on this device */
     RBFX_ST_SINGULAR = 8,        /* zero pivot: system singular                     */
     RBFX_ST_EVAL_NOT_FOUND = 9,  /* Hermite stencil without its eval point
                                     (assembly.jl:35-46 raises ArgumentError)        */
@@ -237,7 +239,7 @@ typedef struct rbfx_spmv_args {
                                     workgroups read HBM directly)                       */
     int32_t mapping;             /* ABI 2: 0 automatic, 1 lanes per row reading HBM,
                                     2 staged through LDS (needs rows stored in processing
-                                    order and F < 8; RBFX_EUNSUPPORTED otherwise)        */
+                                    order; RBFX_EUNSUPPORTED otherwise)                  */
 } rbfx_spmv_args;
 
 /* ---- entry points -------------------------------------------------------- */
@@ -302,8 +304,8 @@ int rbfx_last_build_timing(double ms[4]);
 
 /* Size buckets of the last build on this thread (no reference counterpart): stencils
  * with n = k + m <= 32, 33..64 and 65..RBFX_MAX_N, and the kernel each ran on
- * (0: registers, two stencils per wavefront; 1: registers, one; 2: system in LDS;
- * -1: none). Uneven builds (ragged k or a degree map) classify stencils into these
+ * (0: registers, two stencils per wavefront; 1: registers, one; 2: system in LDS,
+ * FP64; 3, 4: kernels 0, 1 in FP64 within an FP32 + refinement build; -1: none). Uneven builds (ragged k or a degree map) classify stencils into these
  * buckets; the kernel of a bucket is timed on a sample the first time a
  * configuration is seen, and the fastest is kept (counts of -1: not read back).  */
 int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3]);

@@ -39,6 +39,7 @@ ST_DIRICHLET = 1
 ST_REFINED = 2
 ST_FALLBACK64 = 3
 ST_LEARNED = 4
+ST_FP64 = 5
 ST_SINGULAR = 8
 ST_EVAL_NOT_FOUND = 9
 ST_TOO_LARGE = 10

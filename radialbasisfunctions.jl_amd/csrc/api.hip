@@ -190,15 +190,11 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
     if (a->max_row_len < 0) return set_err(RBFX_EINVAL, "rbfx_spmv: max_row_len must be >= 0");
     P.max_row = a->row_ptr ? a->max_row_len : a->k;
     P.mapping = a->mapping;
-    if (a->mapping == 2) {
-        const int64_t L = P.max_row;
-        const int g = L <= 20 ? 4 : (L <= 40 ? 8 : 16);
-        const int64_t lds = (256 / g) * L * (8 * (int64_t)std::min(4, a->n_valarrays) + 4);
-        if ((a->row_order && !a->rows_permuted) || a->n_fields >= 8 || L <= 0 || lds > 48 * 1024)
-            return set_err(RBFX_EUNSUPPORTED,
-                           "rbfx_spmv: staged mapping needs rows stored in processing order, n_fields < 8 and "
-                           "a longest row that fits LDS (got %lld)", (long long)L);
-    }
+    P.y = a->y;
+    if (a->mapping == 2 && !rbfx::spmv_staged_fits(P, std::min(4, a->n_valarrays)))
+        return set_err(RBFX_EUNSUPPORTED,
+                       "rbfx_spmv: staged mapping needs rows stored in processing order and a longest row "
+                       "that fits LDS (got %lld)", (long long)P.max_row);
     for (int v0 = 0; v0 < a->n_valarrays; v0 += 4) {
         const int nv = std::min(4, a->n_valarrays - v0);
         P.n_valarrays = nv;
@@ -213,38 +209,97 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
 
 // ---------------------------------------------------------------- size buckets
 // Kernels a bucket can run on: the register kernel with two stencils per wavefront
-// (n <= 32), the register kernel with one (n <= 64), the LDS kernel (n <= RBFX_MAX_N).
-enum { K_REG32 = 0, K_REG64 = 1, K_WIDE = 2 };
+// (n <= 32), the register kernel with one (n <= 64), the LDS kernel (n <= RBFX_MAX_N,
+// FP64), and, in FP32 + refinement mode, the two register kernels in FP64.
+enum { K_REG32 = 0, K_REG64 = 1, K_WIDE = 2, K_REG32_F64 = 3, K_REG64_F64 = 4, kNumKernels = 5 };
 int g_force_kernel[3] = {-1, -1, -1};  // rbfx_tune_bucket overrides
 thread_local int64_t g_bucket_n[3];
 thread_local int32_t g_bucket_kernel[3];
 constexpr int64_t kTuneMin = 4096;     // stencils in a bucket before its mapping is timed
 constexpr int32_t kTuneSample = 16384;
 
+// In FP32 + refinement mode the FP64 kernels are candidates too (their stencils get
+// status FP64): with packed FP32 arithmetic unused, FP32 issues at the FP64 rate on
+// gfx950 and the FP32 kernels hold extra registers for the refinement, so an FP64
+// solve can be the faster one for a size class; the tuner measures which.
 static bool kernel_serves(int kid, int bucket, bool fp32, int n_cols)
 {
     if (kid == K_REG32) return bucket == 0;
     if (kid == K_REG64) return bucket <= 1;
-    if (kid == K_WIDE) return bucket == 2 || (!fp32 && bucket == 1 && rbfx::wide_fits(64, n_cols));
+    if (kid == K_WIDE) return bucket == 2 || (bucket == 1 && rbfx::wide_fits(64, n_cols));
+    if (kid == K_REG32_F64) return fp32 && bucket == 0;
+    if (kid == K_REG64_F64) return fp32 && bucket <= 1;
     return false;
 }
 
 static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, int dim, bool fp32, int rows_cap, int64_t n_work,
                                    hipStream_t st)
 {
-    if (kid == K_WIDE) {
-        rbfx::BuildDev Q = P;
-        if (fp32) Q.ok_status = RBFX_ST_FALLBACK64;  // the LDS kernel solves in FP64
-        return rbfx::launch_build_wide(Q, dim, rows_cap, n_work, st);
+    rbfx::BuildDev Q = P;
+    if (fp32 && (kid == K_WIDE || kid == K_REG32_F64 || kid == K_REG64_F64)) Q.ok_status = RBFX_ST_FP64;
+    if (kid == K_WIDE) return rbfx::launch_build_wide(Q, dim, rows_cap, n_work, st);
+    const bool f32 = fp32 && (kid == K_REG32 || kid == K_REG64);
+    return rbfx::launch_build(Q, dim, (kid == K_REG32 || kid == K_REG32_F64) ? 32 : 64, f32, n_work, st);
+}
+
+// Kernel for a size bucket: forced (rbfx_tune_bucket), cached for this configuration,
+// or timed on a sample of the bucket's stencils (the autotuner: each kernel that serves
+// the bucket runs the sample twice, the second run timed; the fastest is kept), or the
+// narrowest kernel that fits. Q describes the sample: a work list whose length is
+// sample_len (uneven builds), or the first `sample` stencils in processing order.
+static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, DeviceState& ds, int bucket, bool fp32,
+                       int rows_cap, int64_t count, int32_t* sample_len, bool async, hipStream_t st, int* kid)
+{
+    const int dflt = bucket == 0 ? K_REG32 : (bucket == 1 ? K_REG64 : K_WIDE);
+    *kid = dflt;
+    if (g_force_kernel[bucket] >= 0 && kernel_serves(g_force_kernel[bucket], bucket, fp32, Q.n_cols)) {
+        *kid = g_force_kernel[bucket];
+        return RBFX_OK;
     }
-    return rbfx::launch_build(P, dim, kid == K_REG32 ? 32 : 64, fp32, n_work, st);
+    const int key = ((a->dim * 16 + Q.n_cols) * 4 + a->precision) * 4 + bucket;
+    auto it = ds.tuned.find(key);
+    if (it != ds.tuned.end()) {
+        *kid = it->second;
+        return RBFX_OK;
+    }
+    if (async || count < kTuneMin) return RBFX_OK;
+    int cands[kNumKernels], nc = 0;
+    for (int c = 0; c < kNumKernels; ++c)
+        if (kernel_serves(c, bucket, fp32, Q.n_cols)) cands[nc++] = c;
+    if (nc < 2) {
+        ds.tuned[key] = dflt;
+        return RBFX_OK;
+    }
+    const int32_t sample = (int32_t)std::min<int64_t>(count, kTuneSample);
+    rbfx::BuildDev S = Q;  // the sample: the first `sample` entries of the list, or stencils
+    if (sample_len) {
+        hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, st, sample_len, sample);
+        S.list_len = sample_len;
+    }
+    for (int i = 0; i < 2; ++i)
+        if (!ds.tev[i]) HIPCHK(hipEventCreate(&ds.tev[i]));
+    float best = 0.f;
+    for (int c = 0; c < nc; ++c) {
+        HIPCHK(launch_kernel_id(cands[c], S, a->dim, fp32, rows_cap, sample, st));  // warm-up
+        HIPCHK(hipEventRecord(ds.tev[0], st));
+        HIPCHK(launch_kernel_id(cands[c], S, a->dim, fp32, rows_cap, sample, st));
+        HIPCHK(hipEventRecord(ds.tev[1], st));
+        HIPCHK(hipEventSynchronize(ds.tev[1]));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ds.tev[0], ds.tev[1]));
+        if (c == 0 || ms < best) {
+            best = ms;
+            *kid = cands[c];
+        }
+    }
+    ds.tuned[key] = *kid;
+    return RBFX_OK;
 }
 
 // Main solve of all stencils (plain FP64, or FP32 + refinement). Stencils of one size
 // run in one launch. Uneven stencils (ragged k, per-stencil degree) are classified into
-// size buckets in processing order, and each bucket runs on the kernel chosen for it:
-// forced (rbfx_tune_bucket), or timed once per configuration on a sample of the bucket
-// (the autotuner), or the narrowest kernel that fits.
+// size buckets in processing order, and each bucket runs on the kernel chosen for it.
+// (A tuning sample's queued FP32 failures are discarded with the old count.)
 static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceState& ds, int nmax, bool fp32,
                      bool async, hipStream_t st)
 {
@@ -255,8 +310,10 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceSt
     }
     const bool uneven = (a->adj_ptr || a->poly_deg_map) && nmax > 32;
     if (!uneven) {
-        const int kid = nmax <= 32 ? K_REG32 : (nmax <= 64 ? K_REG64 : K_WIDE);
         const int b = nmax <= 32 ? 0 : (nmax <= 64 ? 1 : 2);
+        int kid = K_REG32;
+        const int rc = pick_kernel(a, P, ds, b, fp32, nmax, a->n_eval, nullptr, async, st, &kid);
+        if (rc != RBFX_OK) return rc;
         g_bucket_n[b] = a->n_eval;
         g_bucket_kernel[b] = kid;
         if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
@@ -282,51 +339,12 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceSt
     const int rows_cap[3] = {32, 64, std::min(nmax, RBFX_MAX_N)};
     int kid[3];
     for (int b = 0; b < 3; ++b) {
-        const int dflt = b == 0 ? K_REG32 : (b == 1 ? K_REG64 : K_WIDE);
-        kid[b] = dflt;
-        if (g_force_kernel[b] >= 0 && kernel_serves(g_force_kernel[b], b, fp32, P.n_cols)) {
-            kid[b] = g_force_kernel[b];
-            continue;
-        }
-        const int key = ((dim * 16 + P.n_cols) * 4 + a->precision) * 4 + b;
-        auto it = ds.tuned.find(key);
-        if (it != ds.tuned.end()) {
-            kid[b] = it->second;
-            continue;
-        }
-        if (async || cnt[b] < kTuneMin) continue;
-        // time each kernel that serves this bucket on a sample of its list
-        int cands[3], nc = 0;
-        for (int c = 0; c < 3; ++c)
-            if (kernel_serves(c, b, fp32, P.n_cols)) cands[nc++] = c;
-        if (nc < 2) {
-            ds.tuned[key] = dflt;
-            continue;
-        }
-        const int32_t sample = std::min<int32_t>(cnt[b], kTuneSample);
-        hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, st, ds.bucket_count + 3 + b, sample);
         rbfx::BuildDev Q = P;
         Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
         Q.list_len = ds.bucket_count + 3 + b;
-        for (int i = 0; i < 2; ++i)
-            if (!ds.tev[i]) HIPCHK(hipEventCreate(&ds.tev[i]));
-        float best = 0.f;
-        for (int c = 0; c < nc; ++c) {
-            HIPCHK(launch_kernel_id(cands[c], Q, dim, fp32, rows_cap[b], sample, st));  // warm-up
-            HIPCHK(hipEventRecord(ds.tev[0], st));
-            HIPCHK(launch_kernel_id(cands[c], Q, dim, fp32, rows_cap[b], sample, st));
-            HIPCHK(hipEventRecord(ds.tev[1], st));
-            HIPCHK(hipEventSynchronize(ds.tev[1]));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, ds.tev[0], ds.tev[1]));
-            if (c == 0 || ms < best) {
-                best = ms;
-                kid[b] = cands[c];
-            }
-        }
-        ds.tuned[key] = kid[b];
+        const int rc = pick_kernel(a, Q, ds, b, fp32, rows_cap[b], cnt[b], ds.bucket_count + 3 + b, async, st, &kid[b]);
+        if (rc != RBFX_OK) return rc;
     }
-    // the real launches (a tuning sample's failures are discarded with the old count)
     if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
     for (int b = 0; b < 3; ++b) {
         g_bucket_n[b] = async ? -1 : cnt[b];
@@ -551,7 +569,21 @@ static int build_impl(const rbfx_build_args* a, bool async)
             }  // asynchronous: the grid covers every stencil, the kernel reads the count
         }
         if (a->precision == RBFX_PREC_LEARNED) {
-            if (n_solve > 0) HIPCHK(rbfx::launch_build(P, a->dim, nmax, true, n_solve, st));
+            // the rejected stencils (one size class) on the kernel picked for it
+            if (!async && !ds.bucket_count) HIPCHK(hipMalloc((void**)&ds.bucket_count, 6 * sizeof(int32_t)));
+            const int b = nmax <= 32 ? 0 : 1;
+            int kid = b == 0 ? K_REG32 : K_REG64;
+            const int rc = pick_kernel(a, P, ds, b, true, nmax, n_solve, ds.bucket_count ? ds.bucket_count + 3 : nullptr,
+                                       async, st, &kid);
+            if (rc != RBFX_OK) return rc;
+            hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);  // drop tuning failures
+            for (int i = 0; i < 3; ++i) {
+                g_bucket_n[i] = 0;
+                g_bucket_kernel[i] = -1;
+            }
+            g_bucket_n[b] = async ? -1 : n_solve;
+            g_bucket_kernel[b] = kid;
+            if (n_solve > 0) HIPCHK(launch_kernel_id(kid, P, a->dim, true, nmax, n_solve, st));
         } else {
             const int rc = solve_all(a, P, ds, nmax, true, async, st);
             if (rc != RBFX_OK) return rc;
@@ -568,7 +600,14 @@ static int build_impl(const rbfx_build_args* a, bool async)
         F.list = ds.fail_list;
         F.list_len = ds.fail_count;
         F.ok_status = RBFX_ST_FALLBACK64;
-        if (nfail > 0) HIPCHK(rbfx::launch_build(F, a->dim, std::min(nmax, 64), false, nfail, st));
+        // (stencils of more than 32 rows re-solve on the LDS kernel: the FP64 register
+        // kernel for up to 64 rows spills)
+        if (nfail > 0) {
+            if (nmax > 32 && rbfx::wide_fits(std::min(nmax, RBFX_MAX_N), F.n_cols))
+                HIPCHK(rbfx::launch_build_wide(F, a->dim, std::min(nmax, RBFX_MAX_N), nfail, st));
+            else
+                HIPCHK(rbfx::launch_build(F, a->dim, std::min(nmax, 64), false, nfail, st));
+        }
     }
     HIPCHK(mark(3));
     if (async) {
@@ -637,7 +676,7 @@ int rbfx_last_build_counts(int64_t counts[16])
 int rbfx_tune_bucket(int32_t bucket, int32_t kernel)
 {
     if (bucket < 0 || bucket > 2) return set_err(RBFX_EINVAL, "bucket must be 0, 1 or 2");
-    if (kernel < -1 || kernel > 2 || (kernel >= 0 && !kernel_serves(kernel, bucket, false, 1)))
+    if (kernel < -1 || kernel >= kNumKernels || (kernel >= 0 && !kernel_serves(kernel, bucket, true, 1)))
         return set_err(RBFX_EINVAL, "kernel %d cannot serve bucket %d", kernel, bucket);
     std::lock_guard<std::mutex> lk(g_mu);
     g_force_kernel[bucket] = kernel;

@@ -740,7 +740,7 @@ k_build(const BuildDev P)
         for (int c = 0; c < RC; ++c) {
             if (c < n_cols) {
                 double x64 = (double)x[c];
-                double rel = 1.0;
+                double rel = 1.0, rel_prev = 2.0;
                 for (int it = 0; it < P.refine_steps; ++it) {
                     s_x[g][li] = x64;
                     __syncthreads();
@@ -755,7 +755,12 @@ k_build(const BuildDev P)
                     const double num = group_max<NMAX>(li < n ? fabs((double)dx[0]) : 0.0);
                     const double den = group_max<NMAX>(li < n ? fabs(x64) : 0.0);
                     rel = den > 0.0 ? num / den : num;
-                    if (__ballot(!(rel <= 0.25 * P.accept_tol)) == 0ull) break;
+                    // a group stops when converged, or when the correction does not
+                    // contract (the FP32 factors are too inaccurate for this system:
+                    // it goes to the FP64 re-solve without spending the remaining steps)
+                    const bool stalled = !(rel < 0.5 * rel_prev) || !(rel < 1.0);
+                    rel_prev = rel;
+                    if (__ballot(!(rel <= 0.25 * P.accept_tol) && !stalled) == 0ull) break;
                 }
                 if (!(rel <= P.accept_tol)) converged = false;
                 if (is_pt) P.out_val[(int64_t)c * P.out_nnz + o0 + pi] = gfail ? __builtin_nan("") : x64;

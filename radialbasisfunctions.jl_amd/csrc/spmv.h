@@ -26,6 +26,8 @@ struct SpmvDev {
 };
 
 hipError_t launch_spmv(const SpmvDev& P, hipStream_t s);
+// whether the staged (LDS) mapping serves this apply with nv value arrays per launch
+bool spmv_staged_fits(const SpmvDev& P, int nv);
 
 // copy fixed-length CSR rows into processing order: out row r = in row row_order[r]
 hipError_t launch_permute_rows(int64_t n_rows, int k, const int32_t* row_order, const int32_t* col, const double* vals,

@@ -36,6 +36,23 @@ __device__ __forceinline__ int64_t xcd_block(int64_t nb_real)
 
 __host__ inline int64_t pad8(int64_t nb) { return (nb + 7) / 8 * 8; }
 
+// Tiles of a launch: the grid (a multiple of 8 workgroups) is dealt to the 8 XCDs
+// round-robin; workgroup b takes tiles of XCD (b % 8)'s contiguous eighth, every
+// (grid / 8)-th one. With one workgroup per tile (the grid used here; fewer, longer-
+// lived workgroups measured slower, since staging and compute then serialise) each
+// workgroup runs one tile.
+struct TileRange {
+    int64_t first, end, stride;
+};
+__device__ __forceinline__ TileRange xcd_tile_range(int64_t n_tiles)
+{
+    const int64_t per_x = gridDim.x >> 3;
+    const int64_t x = blockIdx.x & 7, j = blockIdx.x >> 3;
+    const int64_t tpx = (n_tiles + 7) >> 3;
+    const int64_t lo = x * tpx;
+    return {lo + j, lo + tpx < n_tiles ? lo + tpx : n_tiles, per_x};
+}
+
 template <int NV>
 __global__ __launch_bounds__(256) void k_spmv_fields(const SpmvDev P, int G /* pow2 >= F */, int64_t nb_real)
 {
@@ -236,6 +253,102 @@ __global__ __launch_bounds__(256) void k_spmv_staged(const SpmvDev P, int64_t nb
     rows_gather_reduce<NV, G, FMAX, U>(P, r0, r1, b0, (const int32_t*)s_col, (const double*)s_val, (int64_t)cap);
 }
 
+// Many fields (F >= 8): LPR lanes per row, each owning FPL consecutive fields (FPL = 2:
+// 16-byte gathers and stores, when X and Y hold a point's fields contiguously), so a
+// row's gather of one entry reads its F values from one or two cache lines. Each lane
+// walks its row's entries with U independent gathers in flight and needs no reduction.
+template <int NV, int LPR, int FPL, int U, typename ColPtr, typename ValPtr>
+__device__ __forceinline__ void fields_rows(const SpmvDev& P, int64_t r0, int64_t r1, int64_t b0, ColPtr col,
+                                            ValPtr val, int64_t vstride)
+{
+    const int fl = threadIdx.x % LPR;
+    const int f0 = fl * FPL;
+    const int64_t r = r0 + threadIdx.x / LPR;
+    if (r >= r1 || f0 >= P.n_fields) return;
+    const int64_t row = P.row_order ? (int64_t)P.row_order[r] : r;
+    const int64_t qb = (P.row_ptr ? P.row_ptr[r] : r * (int64_t)P.k) - b0;
+    const int64_t qe = (P.row_ptr ? P.row_ptr[r + 1] : (r + 1) * (int64_t)P.k) - b0;
+    const double* xf = P.x + (int64_t)f0 * P.x_ld_field;
+    double acc[NV][FPL];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int j = 0; j < FPL; ++j) acc[v][j] = 0.0;
+    for (int64_t q0 = qb; q0 < qe; q0 += U) {
+        double xv[U][FPL], w[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t q = q0 + u;
+            const bool ok = q < qe;
+            const int64_t c = ok ? (int64_t)col[q] : 0;
+            if constexpr (FPL == 2) {
+                const f64x2 t = *reinterpret_cast<const f64x2*>(xf + c * P.x_ld_point);
+                xv[u][0] = ok ? t.x : 0.0;
+                xv[u][1] = ok ? t.y : 0.0;
+            } else {
+                xv[u][0] = ok ? xf[c * P.x_ld_point] : 0.0;
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) w[u][v] = ok ? val[v * vstride + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int j = 0; j < FPL; ++j) acc[v][j] += w[u][v] * xv[u][j];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double* yp = P.y + (int64_t)v * P.y_ld_val + row * P.y_ld_point + (int64_t)f0 * P.y_ld_field;
+        if constexpr (FPL == 2) {
+            f64x2 out = {P.alpha * acc[v][0], P.alpha * acc[v][1]};
+            if (P.beta != 0.0) {
+                const f64x2 old = *reinterpret_cast<const f64x2*>(yp);
+                out.x += P.beta * old.x;
+                out.y += P.beta * old.y;
+            }
+            *reinterpret_cast<f64x2*>(yp) = out;
+        } else {
+            const double out = P.alpha * acc[v][0];
+            *yp = (P.beta == 0.0) ? out : out + P.beta * *yp;
+        }
+    }
+}
+
+// The workgroup's rows (256 / LPR, stored back to back) first have their index and
+// value runs copied into LDS, as in k_spmv_staged; a workgroup whose entries exceed
+// the LDS it was given (a longest-row bound that was too small) reads HBM instead.
+template <int NV, int LPR, int FPL, int U>
+__global__ __launch_bounds__(256) void k_spmv_fields_staged(const SpmvDev P, int64_t nb_real, int cap)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+    double* s_val = reinterpret_cast<double*>(s_raw);                        // NV x cap
+    int32_t* s_col = reinterpret_cast<int32_t*>(s_val + (int64_t)NV * cap);  // cap
+    constexpr int R = 256 / LPR;
+    const TileRange tr = xcd_tile_range(nb_real);
+    for (int64_t lb = tr.first; lb < tr.end; lb += tr.stride) {  // uniform over the workgroup
+        const int64_t r0 = lb * R;
+        const int64_t r1 = r0 + R < P.n_rows ? r0 + R : P.n_rows;
+        const int64_t b0 = P.row_ptr ? P.row_ptr[r0] : r0 * (int64_t)P.k;
+        const int64_t b1 = P.row_ptr ? P.row_ptr[r1] : r1 * (int64_t)P.k;
+        const int64_t n_e = b1 - b0;
+        if (n_e > cap) {
+            fields_rows<NV, LPR, FPL, U>(P, r0, r1, 0, P.col_idx, P.vals, P.nnz);
+            continue;  // LDS untouched in this tile
+        }
+        for (int i = threadIdx.x; i < n_e; i += 256) {
+            s_col[i] = __builtin_nontemporal_load(P.col_idx + b0 + i);
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+                s_val[v * cap + i] = __builtin_nontemporal_load(P.vals + (int64_t)v * P.nnz + b0 + i);
+        }
+        __syncthreads();
+        fields_rows<NV, LPR, FPL, U>(P, r0, r1, b0, (const int32_t*)s_col, (const double*)s_val, (int64_t)cap);
+        __syncthreads();  // the next tile overwrites the LDS copy
+    }
+}
+
 // LDS bytes of one staged workgroup
 static int64_t staged_lds(int nv, int G, int64_t max_row) { return (256 / G) * max_row * (8 * (int64_t)nv + 4); }
 constexpr int64_t kStagedLdsMax = 48 * 1024;
@@ -248,7 +361,7 @@ static hipError_t spmv_staged(const SpmvDev& P, hipStream_t s)
     {                                                                                                            \
         const int64_t nbr = (P.n_rows + 256 / (GG) - 1) / (256 / (GG));                                          \
         const int cap = (int)((256 / (GG)) * L);                                                                 \
-        hipLaunchKernelGGL((k_spmv_staged<NV, GG, FMAX, UU>), dim3((unsigned)pad8(nbr)), dim3(256),              \
+        hipLaunchKernelGGL((k_spmv_staged<NV, GG, FMAX, UU>), dim3((unsigned)pad8(nbr)), dim3(256),   \
                            (unsigned)staged_lds(NV, GG, L), s, P, nbr, cap);                                     \
         return hipGetLastError();                                                                                \
     }
@@ -263,15 +376,35 @@ static hipError_t spmv_staged(const SpmvDev& P, hipStream_t s)
 // lanes per row of the staged mapping for the longest row L (as chosen in spmv_staged)
 static int staged_g(int64_t L) { return L <= 20 ? 4 : (L <= 40 ? 8 : 16); }
 
+// many-field layout: two fields per lane when X and Y hold each point's fields
+// contiguously and 16-byte aligned (and F is even); lanes per row a power of 2 >= 4
+static bool fields_pair(const SpmvDev& P)
+{
+    return (P.n_fields % 2) == 0 && P.x_ld_field == 1 && (P.x_ld_point % 2) == 0 && ((uintptr_t)P.x & 15) == 0 &&
+           P.y_ld_field == 1 && (P.y_ld_point % 2) == 0 && (P.y_ld_val % 2) == 0 && ((uintptr_t)P.y & 15) == 0;
+}
+static int fields_lpr(const SpmvDev& P)
+{
+    const int fpl = fields_pair(P) ? 2 : 1;
+    int lpr = 4;
+    while (lpr * fpl < P.n_fields) lpr <<= 1;
+    return lpr;
+}
+
+bool spmv_staged_fits(const SpmvDev& P, int nv)
+{
+    if ((P.row_order && !P.rows_permuted) || P.max_row <= 0 || P.nnz <= 0 || P.n_fields > 64) return false;
+    const int lanes = P.n_fields >= 8 ? fields_lpr(P) : staged_g(P.max_row);
+    return (256 / lanes) * P.max_row * (8 * (int64_t)nv + 4) <= kStagedLdsMax;
+}
+
 template <int NV>
 static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
 {
     const int TB = 256;
     // staged mapping: rows stored in processing order, a known longest row, LDS in budget
-    const bool contiguous = !P.row_order || P.rows_permuted;
-    const bool staged_ok = contiguous && P.n_fields < 8 && P.max_row > 0 && P.nnz > 0 &&
-                           staged_lds(NV, staged_g(P.max_row), P.max_row) <= kStagedLdsMax;
-    if (P.mapping == 2 && !staged_ok) return hipErrorInvalidValue;
+    const bool staged_ok = P.n_fields < 8 && spmv_staged_fits(P, NV);
+    if (P.mapping == 2 && P.n_fields < 8 && !staged_ok) return hipErrorInvalidValue;
     if (staged_ok && P.mapping != 1) {
         if (P.n_fields <= 1) return spmv_staged<NV, 1>(P, s);
         if (P.n_fields <= 4) return spmv_staged<NV, 4>(P, s);
@@ -281,6 +414,32 @@ static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
         int G = 8;
         while (G < P.n_fields) G <<= 1;
         if (G > 64) return hipErrorInvalidValue;
+        const bool pair = fields_pair(P);
+        const int lpr = fields_lpr(P);
+        const int64_t cap = (256 / lpr) * P.max_row;
+        const bool fstaged = spmv_staged_fits(P, NV);
+        if (P.mapping == 2 && !fstaged) return hipErrorInvalidValue;
+        if (fstaged && P.mapping != 1) {
+            const int64_t nbr = (P.n_rows + 256 / lpr - 1) / (256 / lpr);
+            const size_t lds = (size_t)cap * (8 * NV + 4);
+#define RBFX_FSTAGED(LL, FF)                                                                                      \
+    {                                                                                                             \
+        hipLaunchKernelGGL((k_spmv_fields_staged<NV, LL, FF, 10>), dim3((unsigned)pad8(nbr)), dim3(256), lds, s, P, \
+                           nbr, (int)cap);                                                                        \
+        return hipGetLastError();                                                                                 \
+    }
+            if (pair) {
+                if (lpr == 4) RBFX_FSTAGED(4, 2)
+                if (lpr == 8) RBFX_FSTAGED(8, 2)
+                if (lpr == 16) RBFX_FSTAGED(16, 2)
+                RBFX_FSTAGED(32, 2)
+            }
+            if (lpr == 8) RBFX_FSTAGED(8, 1)
+            if (lpr == 16) RBFX_FSTAGED(16, 1)
+            if (lpr == 32) RBFX_FSTAGED(32, 1)
+            RBFX_FSTAGED(64, 1)
+#undef RBFX_FSTAGED
+        }
         const int rows_per_block = TB / G;
         const int64_t grid = (P.n_rows + rows_per_block - 1) / rows_per_block;
         hipLaunchKernelGGL(k_spmv_fields<NV>, dim3((unsigned)pad8(grid)), dim3(TB), 0, s, P, G, grid);

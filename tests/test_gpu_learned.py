@@ -202,7 +202,8 @@ def test_learned_build_contract(RBF, O, gpu, trained):
     ref = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu, adjl=op.adj_idx, precision="fp64")
     st = op.status.cpu().numpy()
     acc = st == RBF._abi.ST_LEARNED
-    assert acc.sum() + (st == RBF._abi.ST_REFINED).sum() + (st == RBF._abi.ST_FALLBACK64).sum() == len(pts)
+    assert acc.sum() + (st == RBF._abi.ST_REFINED).sum() + (st == RBF._abi.ST_FALLBACK64).sum() + \
+        (st == RBF._abi.ST_FP64).sum() == len(pts)
     adj = op.adj_idx.cpu().numpy()
     W = op.vals.cpu().numpy()[0].reshape(-1, 20)
     Wr = ref.vals.cpu().numpy()[0].reshape(-1, 20)
