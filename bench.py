@@ -314,7 +314,8 @@ def config_c5(R, dev, stream):
     by = nnz * (4 + 8) + 8 * 16 * (ds.shape[0] + n_t)
     gbs = by / (apply_ms * 1e-3) / 1e9
     out["apply"] = {"ms": apply_ms, "GB_s": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS, "bytes": by,
-                    "kernel": "k_spmv_fields_staged (a lane per row and field, rows in Morton order, LDS-staged)"}
+                    "traffic": profiled_traffic("k_spmv_fields_staged:c5"),
+                    "kernel": "k_spmv_fields_staged (lanes per row over field pairs, rows in Morton order, LDS-staged)"}
     out["interp_nrmse"] = err
     del op, Y
     # the same regrid after reorder_points on the sources: a source's 16 field values
@@ -545,7 +546,7 @@ def main():
                      "kernel_ms": dom["ms"]},
         "plain_fp64_roofline": {"bound": "valu-fp64", "achieved": fp64_achieved, "peak": FP64_PEAK_TFLOPS,
                                 "unit": "TFLOP/s", "frac": fp64_achieved / FP64_PEAK_TFLOPS,
-                                "traffic": profiled_traffic("k_build<double,32,2,1>"),
+                                "traffic": profiled_traffic("k_build<double,32,2,1>:plain"),
                                 "per_unit_flops": flops_per_stencil(n_sys, 1), "kernel_ms": stage_ms["fp64"][0]},
         "cpu_baseline": cpu,
         "configs": configs,
