@@ -229,7 +229,7 @@ template <typename T, int NMAX, int DIM, int UW>
 struct StencilLds {
     double xi[NMAX][DIM];  // scaled offsets of the stencil points
     double ba[NMAX], bb[NMAX], bn[NMAX][DIM];
-    T prow[2][2][UW];  // pivot rows, double-buffered by step parity
+    T prow[2][2][UW];  // candidate / pivot rows: two slots, double-buffered by step parity
     int bt[NMAX];      // 0 interior, 1 Dirichlet, 2 Neumann/Robin (Hermite stencils)
     int piv[NMAX], blk[NMAX];
 };
@@ -353,6 +353,9 @@ k_build(const BuildDev P)
 
     const int64_t lb = xcd_contiguous_block(P.n_blocks);
     if (lb < 0) return;  // padding block: the whole workgroup leaves
+    // the row slots start at zero: a group that has finished (or never started) applies
+    // its slot rows with zero multipliers, which must leave its registers unchanged
+    for (int j = li; j < 2 * 2 * UW; j += NMAX) (&S.prow[0][0][0])[j] = (T)0;
     const int64_t wi = lb * G + g;
     bool alive = wi < n_work;
     int64_t e = alive ? (P.list ? (int64_t)P.list[wi] : (P.order ? (int64_t)P.order[wi] : wi)) : 0;
@@ -601,12 +604,21 @@ k_build(const BuildDev P)
         if (__ballot(gact != 0ull) == 0ull) break;  // every group of the wave is done
         const bool busy = gact != 0ull;             // this lane's group still factorises
         const int c0 = busy ? (int)__builtin_ctzll(gact) : 0;
-        // column c0 = row c0 by symmetry: each lane reads its own entry
-        const T cv = (T)fabs(group_reg_get<NMAX>(r, c0));
+        const int buf = step & 1;
+        // Candidate rows pass through LDS (two slots per group): row c0 first, which is
+        // column c0 of the active block by symmetry, then the rook candidates. The pivot
+        // row(s) are then already in their slots for the update.
+        if (busy && li == c0) {
+#pragma unroll
+            for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
+        }
+        __syncthreads();
+        int sp = 0, sk = 0;  // slots of rows p and kp
+        const T cv = (T)fabs(S.prow[buf][0][li]);
         const uint32_t ck = group_max_u32<NMAX>((act && li != c0) ? argmax_key(cv, li) : 0u);
         int imax = ck ? 63 - (int)(ck & 63u) : c0;
-        T colmax = ck ? group_bcast<NMAX>(cv, imax) : (T)0;
-        const T absakk = (T)fabs(group_bcast<NMAX>(diag, c0));
+        T colmax = ck ? (T)fabs(S.prow[buf][0][imax]) : (T)0;
+        const T absakk = (T)fabs(S.prow[buf][0][c0]);
         int kstep = 1, p = c0, kp = c0;
         bool need_rook = busy && absakk < alpha * colmax;
         if (busy && !(absakk > (T)0 || colmax > (T)0)) {
@@ -616,21 +628,30 @@ k_build(const BuildDev P)
         // rook iterations: every group takes part in the collective steps
         for (int it = 0; it < 2 * NMAX; ++it) {
             if (__ballot(need_rook) == 0ull) break;
-            const T rv = (T)fabs(group_reg_get<NMAX>(r, imax));
+            const int si = 1 - sp;  // row imax into the slot not holding row p
+            if (need_rook && li == imax) {
+#pragma unroll
+                for (int j = 0; j < UW; ++j) S.prow[buf][si][j] = r[j];
+            }
+            __syncthreads();
+            const T rv = need_rook ? (T)fabs(S.prow[buf][si][li]) : (T)0;
             const uint32_t rk = group_max_u32<NMAX>((need_rook && act && li != imax) ? argmax_key(rv, li) : 0u);
             const int ri = rk ? 63 - (int)(rk & 63u) : imax;
-            const T rowmax = rk ? group_bcast<NMAX>(rv, ri) : (T)0;
-            const T aii = (T)fabs(group_bcast<NMAX>(diag, imax));
             if (need_rook) {
+                const T rowmax = rk ? (T)fabs(S.prow[buf][si][ri]) : (T)0;
+                const T aii = (T)fabs(S.prow[buf][si][imax]);
                 if (!(aii < alpha * rowmax)) {
                     kp = imax;
+                    sk = si;
                     need_rook = false;
                 } else if (p == ri || rowmax <= colmax || it == 2 * NMAX - 1) {
                     kp = imax;
+                    sk = si;
                     kstep = 2;
                     need_rook = false;
                 } else {
                     p = imax;
+                    sp = si;
                     colmax = rowmax;
                     imax = ri;
                 }
@@ -639,20 +660,12 @@ k_build(const BuildDev P)
         // a singular group stops here (failed is uniform over the group)
         const bool go = busy && !failed;
         const int p0 = (kstep == 1) ? kp : p;
-        const int p1 = kp;  // equals p0 for a 1x1 pivot
-        const int buf = step & 1;
+        const int p1 = kp;                      // equals p0 for a 1x1 pivot
+        const T* U0 = S.prow[buf][(kstep == 1) ? sk : sp];
+        const T* U1 = S.prow[buf][sk];          // equals U0 for a 1x1 pivot
         // a wave with a 2x2 pivot in either group runs the two-row update for both;
-        // a 1x1 group then carries its pivot row in both slots with a zero multiplier
+        // a 1x1 group then applies its pivot row twice, the second time with a zero multiplier
         const bool two_any = __ballot(go && kstep == 2) != 0ull;
-        // pivot row(s) to LDS: the broadcast source of the update
-        if (go && li == p0) {
-#pragma unroll
-            for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
-        }
-        if (go && two_any && li == p1) {
-#pragma unroll
-            for (int j = 0; j < UW; ++j) S.prow[buf][1][j] = r[j];
-        }
         if (go && li == 0) {
             S.piv[slot] = p0;
             S.blk[slot] = kstep;
@@ -661,19 +674,18 @@ k_build(const BuildDev P)
                 S.blk[slot + 1] = 0;
             }
         }
-        __syncthreads();
-        const T d0 = group_bcast<NMAX>(diag, p0);
-        const T d1 = group_bcast<NMAX>(diag, p1);
-        const T u0 = S.prow[buf][0][li];
+        const T d0 = U0[p0];
+        const T d1 = U1[p1];
+        const T u0 = U0[li];
         T l0, l1 = (T)0;
         if (kstep == 1) {
             if (go && d0 == (T)0) failed = true;
             l0 = u0 / d0;
         } else {
-            const T bq = S.prow[buf][0][p1];
+            const T bq = U0[p1];
             const T det = d0 * d1 - bq * bq;
             if (go && det == (T)0) failed = true;
-            const T u1 = S.prow[buf][1][li];
+            const T u1 = U1[li];
             const T idet = (T)1 / det;
             l0 = (u0 * d1 - u1 * bq) * idet;
             l1 = (u1 * d0 - u0 * bq) * idet;
@@ -688,10 +700,10 @@ k_build(const BuildDev P)
         // in-place fused multiply-adds keep the register row the only live copy
         const T ml0 = -l0, ml1 = -l1;
         diag = fma(ml0, u0, diag);
-        axpy_row<NMAX, RC>(r, ml0, S.prow[buf][0], n_cols);
+        axpy_row<NMAX, RC>(r, ml0, U0, n_cols);
         if (two_any) {
-            diag = fma(ml1, S.prow[buf][1][li], diag);
-            axpy_row<NMAX, RC>(r, ml1, S.prow[buf][1], n_cols);
+            diag = fma(ml1, U1[li], diag);
+            axpy_row<NMAX, RC>(r, ml1, U1, n_cols);
         }
         if constexpr (REFINE) {
             group_reg_set<NMAX>(Lm, p0, l0, mine);

@@ -144,7 +144,9 @@ __global__ __launch_bounds__(256) void k_spmv_rows(const SpmvDev P, int64_t nb_r
 // G lanes per row: gather X (U independent gathers in flight per lane), accumulate
 // the row's products, butterfly-reduce inside the group and write Y. Entries are read
 // from `col` / `val` (value array v at val + v * vstride) at offsets relative to b0.
-template <int NV, int G, int FMAX, int U, typename ColPtr, typename ValPtr>
+// (IDX: the type of entry offsets, 32-bit inside a staged tile; UNITX: one field with
+// unit point stride, so a gather is x[c])
+template <int NV, int G, int FMAX, int U, bool UNITX, typename IDX, typename ColPtr, typename ValPtr>
 __device__ __forceinline__ void rows_gather_reduce(const SpmvDev& P, int64_t r0, int64_t r1, int64_t b0, ColPtr col,
                                                    ValPtr val, int64_t vstride)
 {
@@ -158,18 +160,23 @@ __device__ __forceinline__ void rows_gather_reduce(const SpmvDev& P, int64_t r0,
 #pragma unroll
         for (int f = 0; f < FMAX; ++f) acc[v][f] = 0.0;
     if (live) {
-        const int64_t qb = (P.row_ptr ? P.row_ptr[r] : r * (int64_t)P.k) - b0;
-        const int64_t qe = (P.row_ptr ? P.row_ptr[r + 1] : (r + 1) * (int64_t)P.k) - b0;
-        for (int64_t q0 = qb + sub; q0 < qe; q0 += U * G) {
+        const IDX qb = (IDX)((P.row_ptr ? P.row_ptr[r] : r * (int64_t)P.k) - b0);
+        const IDX qe = (IDX)((P.row_ptr ? P.row_ptr[r + 1] : (r + 1) * (int64_t)P.k) - b0);
+        for (IDX q0 = qb + sub; q0 < qe; q0 += U * G) {
             double xv[U][FMAX], w[U][NV];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                const int64_t q = q0 + u * G;
+                const IDX q = q0 + u * G;
                 const bool ok = q < qe;
-                const int64_t c = ok ? (int64_t)col[q] : 0;
+                const int32_t c = ok ? col[q] : 0;
+                if constexpr (UNITX) {
+                    xv[u][0] = ok ? P.x[c] : 0.0;
+                } else {
 #pragma unroll
-                for (int f = 0; f < FMAX; ++f)
-                    xv[u][f] = (ok && f < P.n_fields) ? P.x[c * P.x_ld_point + (int64_t)f * P.x_ld_field] : 0.0;
+                    for (int f = 0; f < FMAX; ++f)
+                        xv[u][f] = (ok && f < P.n_fields) ? P.x[(int64_t)c * P.x_ld_point + (int64_t)f * P.x_ld_field]
+                                                          : 0.0;
+                }
 #pragma unroll
                 for (int v = 0; v < NV; ++v) w[u][v] = ok ? val[v * vstride + q] : 0.0;
             }
@@ -207,7 +214,7 @@ __device__ __forceinline__ void rows_gather_reduce(const SpmvDev& P, int64_t r0,
 // value streams reads whole cache lines exactly once; then G lanes per row gather X.
 // A workgroup whose entries exceed the LDS capacity (a longest-row bound that was too
 // small) reads them from HBM instead.
-template <int NV, int G, int FMAX, int U>
+template <int NV, int G, int FMAX, int U, bool UNITX>
 __global__ __launch_bounds__(256) void k_spmv_staged(const SpmvDev P, int64_t nb_real, int cap)
 {
     extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
@@ -222,7 +229,7 @@ __global__ __launch_bounds__(256) void k_spmv_staged(const SpmvDev P, int64_t nb
     const int64_t b1 = P.row_ptr ? P.row_ptr[r1] : r1 * (int64_t)P.k;
     const int64_t n_e = b1 - b0;
     if (n_e > cap) {
-        rows_gather_reduce<NV, G, FMAX, U>(P, r0, r1, 0, P.col_idx, P.vals, P.nnz);
+        rows_gather_reduce<NV, G, FMAX, U, UNITX, int64_t>(P, r0, r1, 0, P.col_idx, P.vals, P.nnz);
         return;
     }
     // 16-byte loads (2 values / 4 indices per lane) where the run and both copies are
@@ -250,7 +257,8 @@ __global__ __launch_bounds__(256) void k_spmv_staged(const SpmvDev P, int64_t nb
         }
     }
     __syncthreads();
-    rows_gather_reduce<NV, G, FMAX, U>(P, r0, r1, b0, (const int32_t*)s_col, (const double*)s_val, (int64_t)cap);
+    rows_gather_reduce<NV, G, FMAX, U, UNITX, int>(P, r0, r1, b0, (const int32_t*)s_col, (const double*)s_val,
+                                                   (int64_t)cap);
 }
 
 // Many fields (F >= 8): LPR lanes per row, each owning FPL consecutive fields (FPL = 2:
@@ -353,7 +361,7 @@ __global__ __launch_bounds__(256) void k_spmv_fields_staged(const SpmvDev P, int
 static int64_t staged_lds(int nv, int G, int64_t max_row) { return (256 / G) * max_row * (8 * (int64_t)nv + 4); }
 constexpr int64_t kStagedLdsMax = 48 * 1024;
 
-template <int NV, int FMAX>
+template <int NV, int FMAX, bool UNITX>
 static hipError_t spmv_staged(const SpmvDev& P, hipStream_t s)
 {
     const int64_t L = P.max_row;
@@ -361,7 +369,7 @@ static hipError_t spmv_staged(const SpmvDev& P, hipStream_t s)
     {                                                                                                            \
         const int64_t nbr = (P.n_rows + 256 / (GG) - 1) / (256 / (GG));                                          \
         const int cap = (int)((256 / (GG)) * L);                                                                 \
-        hipLaunchKernelGGL((k_spmv_staged<NV, GG, FMAX, UU>), dim3((unsigned)pad8(nbr)), dim3(256),   \
+        hipLaunchKernelGGL((k_spmv_staged<NV, GG, FMAX, UU, UNITX>), dim3((unsigned)pad8(nbr)), dim3(256),   \
                            (unsigned)staged_lds(NV, GG, L), s, P, nbr, cap);                                     \
         return hipGetLastError();                                                                                \
     }
@@ -406,9 +414,10 @@ static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
     const bool staged_ok = P.n_fields < 8 && spmv_staged_fits(P, NV);
     if (P.mapping == 2 && P.n_fields < 8 && !staged_ok) return hipErrorInvalidValue;
     if (staged_ok && P.mapping != 1) {
-        if (P.n_fields <= 1) return spmv_staged<NV, 1>(P, s);
-        if (P.n_fields <= 4) return spmv_staged<NV, 4>(P, s);
-        return spmv_staged<NV, 8>(P, s);
+        if (P.n_fields <= 1)
+            return P.x_ld_point == 1 ? spmv_staged<NV, 1, true>(P, s) : spmv_staged<NV, 1, false>(P, s);
+        if (P.n_fields <= 4) return spmv_staged<NV, 4, false>(P, s);
+        return spmv_staged<NV, 8, false>(P, s);
     }
     if (P.n_fields >= 8) {
         int G = 8;
