@@ -471,6 +471,40 @@ def test_graph_capture_build_and_apply(RBF, gpu, precision):
         assert torch.equal(y[0, :, 0], y_ref), rep
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_graph_capture_uneven_build(RBF, gpu, precision):
+    """An uneven build (ragged k 12..60, per-stencil degree 2 or 3, so all three size
+    buckets) captured with its apply: every replay classifies, launches each bucket
+    with upper-bound grids and reproduces the eager weights bit for bit."""
+    import torch
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(18)
+    n = 30_000
+    pts = rng.random((n, 2))
+    ks = rng.integers(12, 61, n)
+    _, full = cKDTree(pts).query(pts, 60)
+    adjl = [full[i, :ks[i]] for i in range(n)]
+    deg = rng.integers(2, 4, n).astype(np.int8)
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=3), adjl=adjl, device=gpu, precision=precision,
+                       poly_deg_map=deg)
+    nb, _ = RBF._lib.last_build_buckets()
+    assert all(c > 0 for c in nb)
+    w_eager = op.vals.clone()
+    f0 = np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1])
+    G = op.capture(f0)
+    for rep in range(3):
+        f = f0 * (1.0 + 0.5 * rep) + rep * pts[:, 0]
+        G.x.copy_(torch.as_tensor(f, device=gpu)[:, None])
+        op.vals.fill_(0.0)
+        torch.cuda.synchronize()
+        y = G.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(op.vals, w_eager), rep
+        y_ref = op(torch.as_tensor(f, device=gpu))
+        assert torch.equal(y[0, :, 0], y_ref), rep
+
+
 # ------------------------------------------------ config-shaped cases (reduced N)
 def _raw_build(RBF, gpu, pts, adj, basis, n_cols, terms, precision=0):
     """One build through the C ABI with an arbitrary term table (fused RHS columns)."""
