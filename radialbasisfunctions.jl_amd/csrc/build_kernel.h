@@ -257,17 +257,24 @@ __device__ __forceinline__ void back_substitute(const T (&r)[N], const int* s_pi
         const bool one = blk == 1, two = (blk == 0) && (bprev == 2);
         const int p = one ? s_piv[s] : (two ? s_piv[s - 1] : 0);
         const int q = two ? s_piv[s] : 0;
+        // the second row of a 2x2 block only when one of the wave's groups has one here
+        const bool two_any = __ballot(two) != 0ull;
         const T up = group_reg_get<NMAX>(r, p);
-        const T uq = group_reg_get<NMAX>(r, q);
-        const T dp = group_bcast<NMAX>(diag, p), dq = group_bcast<NMAX>(diag, q);
-        const T bq = group_bcast<NMAX>(uq, p);  // U[p][q]
-        const T idet = (T)1 / (dp * dq - bq * bq);
+        T uq = (T)0, dq = (T)1, bq = (T)0, idet = (T)1;
+        const T dp = group_bcast<NMAX>(diag, p);
+        if (two_any) {
+            uq = group_reg_get<NMAX>(r, q);
+            dq = group_bcast<NMAX>(diag, q);
+            bq = group_bcast<NMAX>(uq, p);  // U[p][q]
+            idet = (T)1 / (dp * dq - bq * bq);
+        }
 #pragma unroll
         for (int c = 0; c < RC; ++c) {
             if (c < n_cols) {
                 const T res = y[c] - acc[c];
                 const T xp = group_bcast<NMAX>(res * dinv, p);
-                const T y0 = group_bcast<NMAX>(res, p), y1 = group_bcast<NMAX>(res, q);
+                const T y0 = two_any ? group_bcast<NMAX>(res, p) : (T)0;
+                const T y1 = two_any ? group_bcast<NMAX>(res, q) : (T)0;
                 const T x0 = two ? (y0 * dq - y1 * bq) * idet : xp;
                 const T x1 = two ? (y1 * dp - y0 * bq) * idet : (T)0;
                 x[c] = ((one || two) && li == p) ? x0 : ((two && li == q) ? x1 : x[c]);
@@ -288,8 +295,9 @@ __device__ __forceinline__ T forward_replay(const T (&Lm)[N], const int* s_piv, 
         const int blk = s < nslots ? s_blk[s] : -1;
         const int p = blk >= 1 ? s_piv[s] : 0;
         const int q = blk == 2 ? s_piv[s + 1] : 0;
-        const T y0 = group_bcast<NMAX>(y, p), y1 = group_bcast<NMAX>(y, q);
-        const T l0 = group_reg_get<NMAX>(Lm, p), l1 = group_reg_get<NMAX>(Lm, q);
+        const bool two_any = __ballot(blk == 2) != 0ull;
+        const T y0 = group_bcast<NMAX>(y, p), y1 = two_any ? group_bcast<NMAX>(y, q) : (T)0;
+        const T l0 = group_reg_get<NMAX>(Lm, p), l1 = two_any ? group_reg_get<NMAX>(Lm, q) : (T)0;
         if (blk == 1 && myslot > s) y -= l0 * y0;
         if (blk == 2 && myslot > s + 1) y -= l0 * y0 + l1 * y1;
     }
