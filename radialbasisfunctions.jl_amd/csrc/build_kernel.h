@@ -260,9 +260,9 @@ __device__ __forceinline__ void back_substitute(const T (&r)[N], const int* s_pi
         // the second row of a 2x2 block only when one of the wave's groups has one here
         const bool two_any = __ballot(two) != 0ull;
         const T up = group_reg_get<NMAX>(r, p);
-        T uq = (T)0, dq = (T)1, bq = (T)0, idet = (T)1;
-        const T dp = group_bcast<NMAX>(diag, p);
+        T uq = (T)0, dp = (T)1, dq = (T)1, bq = (T)0, idet = (T)1;
         if (two_any) {
+            dp = group_bcast<NMAX>(diag, p);
             uq = group_reg_get<NMAX>(r, q);
             dq = group_bcast<NMAX>(diag, q);
             bq = group_bcast<NMAX>(uq, p);  // U[p][q]
@@ -715,7 +715,7 @@ k_build(const BuildDev P)
         }
         if constexpr (REFINE) {
             group_reg_set<NMAX>(Lm, p0, l0, mine);
-            group_reg_set<NMAX>(Lm, p1, l1, mine && kstep == 2);
+            if (two_any) group_reg_set<NMAX>(Lm, p1, l1, mine && kstep == 2);
         }
         if (go && !failed) {
             if (li == p0) {
