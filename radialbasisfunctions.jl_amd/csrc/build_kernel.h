@@ -341,6 +341,9 @@ k_build(const BuildDev P)
     constexpr int G = 64 / NMAX;   // stencils per wavefront
     constexpr int UW = NMAX + RC;  // register row: matrix entries + right-hand sides
     constexpr int NL = REFINE ? NMAX : 1;
+    // two groups per wavefront select registers by a per-group (not wave-uniform) index:
+    // candidate rows then pass through LDS instead of register switches
+    constexpr bool SLOT_ROWS = G > 1;
 
     __shared__ StencilLds<T, NMAX, DIM, UW> s_st[G];
     __shared__ double s_x[G][NL];
@@ -613,63 +616,116 @@ k_build(const BuildDev P)
         const bool busy = gact != 0ull;             // this lane's group still factorises
         const int c0 = busy ? (int)__builtin_ctzll(gact) : 0;
         const int buf = step & 1;
-        // Candidate rows pass through LDS (two slots per group): row c0 first, which is
-        // column c0 of the active block by symmetry, then the rook candidates. The pivot
-        // row(s) are then already in their slots for the update.
-        if (busy && li == c0) {
-#pragma unroll
-            for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
-        }
-        __syncthreads();
-        int sp = 0, sk = 0;  // slots of rows p and kp
-        const T cv = (T)fabs(S.prow[buf][0][li]);
-        const uint32_t ck = group_max_u32<NMAX>((act && li != c0) ? argmax_key(cv, li) : 0u);
-        int imax = ck ? 63 - (int)(ck & 63u) : c0;
-        T colmax = ck ? (T)fabs(S.prow[buf][0][imax]) : (T)0;
-        const T absakk = (T)fabs(S.prow[buf][0][c0]);
         int kstep = 1, p = c0, kp = c0;
-        bool need_rook = busy && absakk < alpha * colmax;
-        if (busy && !(absakk > (T)0 || colmax > (T)0)) {
-            failed = true;  // zero column: singular
-            need_rook = false;
-        }
-        // rook iterations: every group takes part in the collective steps
-        for (int it = 0; it < 2 * NMAX; ++it) {
-            if (__ballot(need_rook) == 0ull) break;
-            const int si = 1 - sp;  // row imax into the slot not holding row p
-            if (need_rook && li == imax) {
+        int sp = 0, sk = 0;  // LDS slots of rows p and kp
+        bool need_rook;
+        if constexpr (SLOT_ROWS) {
+            // Candidate rows pass through LDS (two slots per group): row c0 first, which is
+            // column c0 of the active block by symmetry, then the rook candidates. The pivot
+            // row(s) are then already in their slots for the update.
+            if (busy && li == c0) {
 #pragma unroll
-                for (int j = 0; j < UW; ++j) S.prow[buf][si][j] = r[j];
+                for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
             }
             __syncthreads();
-            const T rv = need_rook ? (T)fabs(S.prow[buf][si][li]) : (T)0;
-            const uint32_t rk = group_max_u32<NMAX>((need_rook && act && li != imax) ? argmax_key(rv, li) : 0u);
-            const int ri = rk ? 63 - (int)(rk & 63u) : imax;
-            if (need_rook) {
-                const T rowmax = rk ? (T)fabs(S.prow[buf][si][ri]) : (T)0;
-                const T aii = (T)fabs(S.prow[buf][si][imax]);
-                if (!(aii < alpha * rowmax)) {
-                    kp = imax;
-                    sk = si;
-                    need_rook = false;
-                } else if (p == ri || rowmax <= colmax || it == 2 * NMAX - 1) {
-                    kp = imax;
-                    sk = si;
-                    kstep = 2;
-                    need_rook = false;
-                } else {
-                    p = imax;
-                    sp = si;
-                    colmax = rowmax;
-                    imax = ri;
+            const T cv = (T)fabs(S.prow[buf][0][li]);
+            const uint32_t ck = group_max_u32<NMAX>((act && li != c0) ? argmax_key(cv, li) : 0u);
+            int imax = ck ? 63 - (int)(ck & 63u) : c0;
+            T colmax = ck ? (T)fabs(S.prow[buf][0][imax]) : (T)0;
+            const T absakk = (T)fabs(S.prow[buf][0][c0]);
+            need_rook = busy && absakk < alpha * colmax;
+            if (busy && !(absakk > (T)0 || colmax > (T)0)) {
+                failed = true;  // zero column: singular
+                need_rook = false;
+            }
+            // rook iterations: every group takes part in the collective steps
+            for (int it = 0; it < 2 * NMAX; ++it) {
+                if (__ballot(need_rook) == 0ull) break;
+                const int si = 1 - sp;  // row imax into the slot not holding row p
+                if (need_rook && li == imax) {
+#pragma unroll
+                    for (int j = 0; j < UW; ++j) S.prow[buf][si][j] = r[j];
+                }
+                __syncthreads();
+                const T rv = need_rook ? (T)fabs(S.prow[buf][si][li]) : (T)0;
+                const uint32_t rk = group_max_u32<NMAX>((need_rook && act && li != imax) ? argmax_key(rv, li) : 0u);
+                const int ri = rk ? 63 - (int)(rk & 63u) : imax;
+                if (need_rook) {
+                    const T rowmax = rk ? (T)fabs(S.prow[buf][si][ri]) : (T)0;
+                    const T aii = (T)fabs(S.prow[buf][si][imax]);
+                    if (!(aii < alpha * rowmax)) {
+                        kp = imax;
+                        sk = si;
+                        need_rook = false;
+                    } else if (p == ri || rowmax <= colmax || it == 2 * NMAX - 1) {
+                        kp = imax;
+                        sk = si;
+                        kstep = 2;
+                        need_rook = false;
+                    } else {
+                        p = imax;
+                        sp = si;
+                        colmax = rowmax;
+                        imax = ri;
+                    }
                 }
             }
+        } else {
+            // one group per wavefront: the selection index is wave-uniform, so a column
+            // entry is a scalar-indexed register read and a value a single lane read
+            const T cv = (T)fabs(group_reg_get<NMAX>(r, c0));  // column c0 = row c0 by symmetry
+            const uint32_t ck = group_max_u32<NMAX>((act && li != c0) ? argmax_key(cv, li) : 0u);
+            int imax = ck ? 63 - (int)(ck & 63u) : c0;
+            T colmax = ck ? group_bcast<NMAX>(cv, imax) : (T)0;
+            const T absakk = (T)fabs(group_bcast<NMAX>(diag, c0));
+            need_rook = busy && absakk < alpha * colmax;
+            if (busy && !(absakk > (T)0 || colmax > (T)0)) {
+                failed = true;  // zero column: singular
+                need_rook = false;
+            }
+            for (int it = 0; it < 2 * NMAX; ++it) {
+                if (__ballot(need_rook) == 0ull) break;
+                const T rv = (T)fabs(group_reg_get<NMAX>(r, imax));
+                const uint32_t rk = group_max_u32<NMAX>((need_rook && act && li != imax) ? argmax_key(rv, li) : 0u);
+                const int ri = rk ? 63 - (int)(rk & 63u) : imax;
+                const T rowmax = rk ? group_bcast<NMAX>(rv, ri) : (T)0;
+                const T aii = (T)fabs(group_bcast<NMAX>(diag, imax));
+                if (need_rook) {
+                    if (!(aii < alpha * rowmax)) {
+                        kp = imax;
+                        need_rook = false;
+                    } else if (p == ri || rowmax <= colmax || it == 2 * NMAX - 1) {
+                        kp = imax;
+                        kstep = 2;
+                        need_rook = false;
+                    } else {
+                        p = imax;
+                        colmax = rowmax;
+                        imax = ri;
+                    }
+                }
+            }
+            // the pivot row(s) to LDS, the broadcast source of the update: row p0 in slot 0,
+            // row p1 in slot 1 (which repeats row p0 for a 1x1 pivot)
+            const bool two = __ballot(busy && !failed && kstep == 2) != 0ull;
+            const int p0r = (kstep == 1) ? kp : p;
+            if (busy && !failed && li == p0r) {
+#pragma unroll
+                for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
+            }
+            if (two && busy && !failed && li == kp) {
+#pragma unroll
+                for (int j = 0; j < UW; ++j) S.prow[buf][1][j] = r[j];
+            }
+            __syncthreads();
+            sp = 0;
+            sk = two ? 1 : 0;
         }
         // a singular group stops here (failed is uniform over the group)
         const bool go = busy && !failed;
         const int p0 = (kstep == 1) ? kp : p;
         const int p1 = kp;                      // equals p0 for a 1x1 pivot
-        const T* U0 = S.prow[buf][(kstep == 1) ? sk : sp];
+        const T* U0 = S.prow[buf][(kstep == 1 && SLOT_ROWS) ? sk : sp];
         const T* U1 = S.prow[buf][sk];          // equals U0 for a 1x1 pivot
         // a wave with a 2x2 pivot in either group runs the two-row update for both;
         // a 1x1 group then applies its pivot row twice, the second time with a zero multiplier
