@@ -263,9 +263,13 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, Device
         return RBFX_OK;
     }
     if (async || count < kTuneMin) return RBFX_OK;
+    // FP32-factorising candidates last: their time is charged with the FP64 re-solve of
+    // the sample stencils they fail on, priced at the fastest FP64 candidate's rate
+    const auto f32_factor = [&](int c) { return fp32 && (c == K_REG32 || c == K_REG64); };
     int cands[kNumKernels], nc = 0;
-    for (int c = 0; c < kNumKernels; ++c)
-        if (kernel_serves(c, bucket, fp32, Q.n_cols)) cands[nc++] = c;
+    for (int pass = 0; pass < 2; ++pass)
+        for (int c = 0; c < kNumKernels; ++c)
+            if (kernel_serves(c, bucket, fp32, Q.n_cols) && f32_factor(c) == (pass == 1)) cands[nc++] = c;
     if (nc < 2) {
         ds.tuned[key] = dflt;
         return RBFX_OK;
@@ -278,15 +282,26 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, Device
     }
     for (int i = 0; i < 2; ++i)
         if (!ds.tev[i]) HIPCHK(hipEventCreate(&ds.tev[i]));
-    float best = 0.f;
+    float best = 0.f, f64_per_stencil = -1.f;
     for (int c = 0; c < nc; ++c) {
+        const bool f32c = f32_factor(cands[c]) && Q.fail_count;
         HIPCHK(launch_kernel_id(cands[c], S, a->dim, fp32, rows_cap, sample, st));  // warm-up
+        if (f32c) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, Q.fail_count);
         HIPCHK(hipEventRecord(ds.tev[0], st));
         HIPCHK(launch_kernel_id(cands[c], S, a->dim, fp32, rows_cap, sample, st));
         HIPCHK(hipEventRecord(ds.tev[1], st));
         HIPCHK(hipEventSynchronize(ds.tev[1]));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ds.tev[0], ds.tev[1]));
+        if (f32c) {
+            int32_t fails = 0;
+            HIPCHK(hipMemcpy(&fails, Q.fail_count, sizeof(fails), hipMemcpyDeviceToHost));
+            const float rate = f64_per_stencil >= 0.f ? f64_per_stencil : ms / (float)sample;
+            ms += (float)std::min(fails, sample) * rate;
+        } else {
+            const float r = ms / (float)sample;
+            if (f64_per_stencil < 0.f || r < f64_per_stencil) f64_per_stencil = r;
+        }
         if (c == 0 || ms < best) {
             best = ms;
             *kid = cands[c];
