@@ -239,7 +239,9 @@ typedef struct rbfx_spmv_args {
                                     workgroups read HBM directly)                       */
     int32_t mapping;             /* ABI 2: 0 automatic, 1 lanes per row reading HBM,
                                     2 staged through LDS (needs rows stored in processing
-                                    order; RBFX_EUNSUPPORTED otherwise)                  */
+                                    order), 3 vector loads per lane (needs rows of one
+                                    length, no row_ptr, fewer than 8 fields);
+                                    RBFX_EUNSUPPORTED when the mapping does not serve   */
 } rbfx_spmv_args;
 
 /* ---- entry points -------------------------------------------------------- */

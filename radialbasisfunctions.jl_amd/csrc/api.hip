@@ -186,11 +186,21 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
     P.row_order = a->row_order;
     P.rows_permuted = a->rows_permuted;
     if (a->rows_permuted && !a->row_order) return set_err(RBFX_EINVAL, "rbfx_spmv: rows_permuted needs row_order");
-    if (a->mapping < 0 || a->mapping > 2) return set_err(RBFX_EINVAL, "rbfx_spmv: mapping must be 0, 1 or 2");
+    if (a->mapping < 0 || a->mapping > 3) return set_err(RBFX_EINVAL, "rbfx_spmv: mapping must be 0, 1, 2 or 3");
     if (a->max_row_len < 0) return set_err(RBFX_EINVAL, "rbfx_spmv: max_row_len must be >= 0");
     P.max_row = a->row_ptr ? a->max_row_len : a->k;
     P.mapping = a->mapping;
     P.y = a->y;
+    if (a->mapping == 3) {
+        for (int v0 = 0; v0 < a->n_valarrays; v0 += 4) {
+            rbfx::SpmvDev Q = P;
+            Q.vals = a->vals + (int64_t)v0 * P.nnz;
+            if (!rbfx::vec_entries(Q, std::min(4, a->n_valarrays - v0)))
+                return set_err(RBFX_EUNSUPPORTED,
+                               "rbfx_spmv: vector mapping needs rows of one length (no row_ptr), fewer than 8 "
+                               "fields and at most 64 lanes per row");
+        }
+    }
     if (a->mapping == 2 && !rbfx::spmv_staged_fits(P, std::min(4, a->n_valarrays)))
         return set_err(RBFX_EUNSUPPORTED,
                        "rbfx_spmv: staged mapping needs rows stored in processing order and a longest row "

@@ -7,12 +7,14 @@
 // values) and several value arrays that share one stencil set (gradient components,
 // Laplacian) are applied in one pass, so the index array is read once.
 //
-// Two mappings, both HBM-bound:
+// Mappings, all HBM-bound:
 //   - many fields (F >= 8): one lane per (row, field); the F lanes of a row read the
 //     same index/value (one cache line serves the group) and gather X rows
 //     (point-major X makes each gather one contiguous F*8-byte segment);
 //   - few fields: G lanes per row split the row's nonzeros, each keeps F partial
-//     sums per value array, then a butterfly reduction inside the group.
+//     sums per value array, then a reduction inside the group; rows of one length are
+//     read with vector loads straight from HBM (k_spmv_vec), CSR rows of varying
+//     length are staged through LDS (k_spmv_staged).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -384,6 +386,149 @@ static hipError_t spmv_staged(const SpmvDev& P, hipStream_t s)
 // lanes per row of the staged mapping for the longest row L (as chosen in spmv_staged)
 static int staged_g(int64_t L) { return L <= 20 ? 4 : (L <= 40 ? 8 : 16); }
 
+// Vector mapping, for rows of one length k (no row_ptr) and few fields: G = ceil(k / E)
+// lanes per row, each owning E consecutive entries, which it reads with one 4E-byte
+// index load and E/2 16-byte value loads per value array (E = 4 or 2; E = 1: scalar
+// loads). A wavefront holds floor(64 / G) rows, so its loads cover one contiguous
+// run of rows and read every byte of it once, without an LDS round trip. The G
+// partial sums of a row are added by the row's first lane.
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+template <int NV, int E, int FMAX, bool UNITX, bool PERM>
+__global__ __launch_bounds__(256) void k_spmv_vec(const SpmvDev P, int G, int64_t nb_real)
+{
+    const int64_t lb = xcd_block(nb_real);
+    if (lb < 0) return;
+    const int lane = threadIdx.x & 63;
+    const int rw = 64 / G;  // rows per wavefront
+    const int rl = lane / G, sub = lane - rl * G;
+    const int64_t r = (lb * 4 + (threadIdx.x >> 6)) * rw + rl;
+    const bool live = rl < rw && r < P.n_rows;
+    const int k = P.k;
+    double acc[NV][FMAX];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int f = 0; f < FMAX; ++f) acc[v][f] = 0.0;
+    if (live) {
+        // PERM: the entries are stored in processing order, so their addresses do not
+        // wait for the row_order read (done after the gathers are issued)
+        const int64_t srow = PERM ? r : (P.row_order ? (int64_t)P.row_order[r] : r);
+        const int j0 = sub * E;
+        const int64_t q = srow * (int64_t)k + j0;
+        int c[E];
+        double w[NV][E];
+        if constexpr (E == 4) {
+            const i32x4 c4 = *reinterpret_cast<const i32x4*>(P.col_idx + q);
+            c[0] = c4.x; c[1] = c4.y; c[2] = c4.z; c[3] = c4.w;
+        } else if constexpr (E == 2) {
+            const i32x2 c2 = *reinterpret_cast<const i32x2*>(P.col_idx + q);
+            c[0] = c2.x; c[1] = c2.y;
+        } else {
+            c[0] = P.col_idx[q];
+        }
+#pragma unroll
+        for (int v = 0; v < NV; ++v) {
+            const double* vp = P.vals + (int64_t)v * P.nnz + q;
+            if constexpr (E >= 2) {
+#pragma unroll
+                for (int h = 0; h < E / 2; ++h) {
+                    const f64x2 t = *reinterpret_cast<const f64x2*>(vp + 2 * h);
+                    w[v][2 * h] = t.x;
+                    w[v][2 * h + 1] = t.y;
+                }
+            } else {
+                w[v][0] = vp[0];
+            }
+        }
+        // E divides k (E = 1, or k a multiple of 2 or 4), so every lane's E entries
+        // lie in its row: the gathers are issued unconditionally, all in flight at once
+        double xv[E][FMAX];
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            if constexpr (UNITX) {
+                xv[e][0] = P.x[c[e]];
+            } else {
+#pragma unroll
+                for (int f = 0; f < FMAX; ++f)
+                    xv[e][f] = f < P.n_fields ? P.x[(int64_t)c[e] * P.x_ld_point + (int64_t)f * P.x_ld_field] : 0.0;
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < E; ++e)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int f = 0; f < FMAX; ++f) acc[v][f] += w[v][e] * xv[e][f];
+    }
+    // segmented tree sum over the row's G lanes: at distance o = 1, 2, 4, ... each lane
+    // adds the partial of the lane o places on when that lane is in the same row, so
+    // the row's first lane ends with the whole sum (lanes past the last row of the
+    // wavefront read wrapped lanes; their sums are not used)
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        if (o >= G) break;  // uniform
+        const bool in = sub + o < G;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int f = 0; f < FMAX; ++f) {
+                const double t = __shfl_down(acc[v][f], o, 64);
+                acc[v][f] += in ? t : 0.0;
+            }
+    }
+    if (live && sub == 0) {
+        const int64_t row = P.row_order ? (int64_t)P.row_order[r] : r;
+#pragma unroll
+        for (int v = 0; v < NV; ++v)
+#pragma unroll
+            for (int f = 0; f < FMAX; ++f) {
+                if (f < P.n_fields) {
+                    double* yp = P.y + (int64_t)v * P.y_ld_val + row * P.y_ld_point + (int64_t)f * P.y_ld_field;
+                    const double out = P.alpha * acc[v][f];
+                    *yp = (P.beta == 0.0) ? out : out + P.beta * *yp;
+                }
+            }
+    }
+}
+
+// entries per lane of the vector mapping (0: the mapping does not serve this apply):
+// rows of one length whose runs keep the index and value loads aligned
+int vec_entries(const SpmvDev& P, int nv)
+{
+    if (P.row_ptr || P.k < 1 || P.n_fields >= 8) return 0;
+    bool a16 = ((uintptr_t)P.col_idx & 15) == 0, a8 = ((uintptr_t)P.col_idx & 7) == 0;
+    bool v16 = true;
+    for (int v = 0; v < nv; ++v) v16 = v16 && ((uintptr_t)(P.vals + (int64_t)v * P.nnz) & 15) == 0;
+    int e = 1;
+    if (P.k % 4 == 0 && a16 && v16) e = 4;
+    else if (P.k % 2 == 0 && a8 && v16) e = 2;
+    return (P.k + e - 1) / e <= 64 ? e : 0;
+}
+
+template <int NV, int FMAX, bool UNITX, bool PERM>
+static hipError_t spmv_vec_perm(const SpmvDev& P, int e, hipStream_t s)
+{
+    const int G = (P.k + e - 1) / e;
+    const int64_t rows_per_block = 4 * (64 / G);
+    const int64_t nbr = (P.n_rows + rows_per_block - 1) / rows_per_block;
+    const dim3 grid((unsigned)pad8(nbr));
+    if (e == 4)
+        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM>), grid, dim3(256), 0, s, P, G, nbr);
+    else if (e == 2)
+        hipLaunchKernelGGL((k_spmv_vec<NV, 2, FMAX, UNITX, PERM>), grid, dim3(256), 0, s, P, G, nbr);
+    else
+        hipLaunchKernelGGL((k_spmv_vec<NV, 1, FMAX, UNITX, PERM>), grid, dim3(256), 0, s, P, G, nbr);
+    return hipGetLastError();
+}
+
+template <int NV, int FMAX, bool UNITX>
+static hipError_t spmv_vec(const SpmvDev& P, int e, hipStream_t s)
+{
+    return (P.rows_permuted || !P.row_order) ? spmv_vec_perm<NV, FMAX, UNITX, true>(P, e, s)
+                                             : spmv_vec_perm<NV, FMAX, UNITX, false>(P, e, s);
+}
+
 // many-field layout: two fields per lane when X and Y hold each point's fields
 // contiguously and 16-byte aligned (and F is even); lanes per row a power of 2 >= 4
 static bool fields_pair(const SpmvDev& P)
@@ -410,6 +555,14 @@ template <int NV>
 static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
 {
     const int TB = 256;
+    // vector mapping: rows of one length, few fields (the automatic choice for them)
+    const int ve = vec_entries(P, NV);
+    if (P.mapping == 3 && !ve) return hipErrorInvalidValue;
+    if (ve && (P.mapping == 0 || P.mapping == 3)) {
+        if (P.n_fields <= 1) return P.x_ld_point == 1 ? spmv_vec<NV, 1, true>(P, ve, s) : spmv_vec<NV, 1, false>(P, ve, s);
+        if (P.n_fields <= 4) return spmv_vec<NV, 4, false>(P, ve, s);
+        return spmv_vec<NV, 8, false>(P, ve, s);
+    }
     // staged mapping: rows stored in processing order, a known longest row, LDS in budget
     const bool staged_ok = P.n_fields < 8 && spmv_staged_fits(P, NV);
     if (P.mapping == 2 && P.n_fields < 8 && !staged_ok) return hipErrorInvalidValue;

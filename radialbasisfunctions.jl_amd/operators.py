@@ -417,7 +417,7 @@ class RadialBasisOperator:
     scipy.sparse matrices (a tuple for rank >= 1)."""
 
     SPATIAL_ORDER_MIN = 65536  # below this the X vector sits in L2 anyway
-    SPMV_MAPPING = 0  # rbfx_spmv_args.mapping: 0 automatic, 1 lanes per row, 2 staged through LDS
+    SPMV_MAPPING = 0  # rbfx_spmv_args.mapping: 0 automatic, 1 lanes per row, 2 staged through LDS, 3 vector loads
 
     def __init__(self, L, data, eval_points=None, basis=None, k=None, adjl=None, hermite=None, device=None,
                  precision="fp64", refine_steps=4, accept_tol=1e-12, model=None, poly_deg_map=None, _weights=None):
@@ -645,6 +645,13 @@ class RadialBasisOperator:
     # ---------------------------------------------------------------- apply
     def _spmv(self, X, out=None, alpha=1.0, beta=0.0, asynchronous=False):
         """X: (n_data, F) float64 cuda, point-major. Returns (n_cols, n_eval, F)."""
+        s, Y = self._spmv_args(X, out, alpha, beta, asynchronous)
+        _lib.spmv(s, self.dev, asynchronous=asynchronous)
+        return Y
+
+    def _spmv_args(self, X, out=None, alpha=1.0, beta=0.0, asynchronous=False):
+        """The rbfx_spmv arguments of Y = alpha * W X + beta * Y (and Y): a caller that
+        applies the same operator to the same buffers many times can reuse them."""
         t = _lib.torch()
         F = X.shape[1]
         Y = out if out is not None else t.empty((self.n_cols, self.n_eval, F), dtype=t.float64, device=self.dev)
@@ -685,8 +692,7 @@ class RadialBasisOperator:
             s.col_idx = _lib.ptr(self._plan[0])
             s.vals = _lib.ptr(self._plan[1])
             s.rows_permuted = 1
-        _lib.spmv(s, self.dev, asynchronous=asynchronous)
-        return Y
+        return s, Y
 
     def __call__(self, x, y=None):
         if not self.valid_cache:

@@ -402,11 +402,13 @@ def test_reorder_points(RBF, gpu):
 
 @pytest.mark.parametrize("fields", [1, 3, 6, 16, 40])
 def test_spmv_mappings_agree(RBF, gpu, fields):
-    """Both apply mappings (lanes per row reading HBM; rows staged through LDS) and the
-    automatic choice give the CSR product to summation-order rounding: fixed-k rows in
-    spatial order (apply plan), and variable-length CSR rows with an exact, a too small
-    and a far too small longest-row bound (workgroups whose entries do not fit the LDS
-    they were given read HBM directly)."""
+    """The apply mappings (lanes per row reading HBM; rows staged through LDS; vector
+    loads per lane for rows of one length) and the automatic choice give the CSR
+    product to summation-order rounding: fixed-k rows in spatial order (apply plan)
+    with k = 20, 21 and 30 (4, 1 and 2 entries per lane), and variable-length CSR rows
+    with an exact, a too small and a far too small longest-row bound (workgroups whose
+    entries do not fit the LDS they were given read HBM directly). The vector mapping
+    refuses what it does not serve (8 or more fields, rows of varying length)."""
     import torch
     from scipy.spatial import cKDTree
 
@@ -418,8 +420,12 @@ def test_spmv_mappings_agree(RBF, gpu, fields):
     W = op.weights()
     ref = W @ Fh
     bound = 1e-13 * (abs(W) @ np.abs(Fh))  # summation-order rounding of a k-term dot product
-    for m in (0, 1, 2):
+    for m in (0, 1, 2, 3):
         op.SPMV_MAPPING = m
+        if m == 3 and fields >= 8:
+            with pytest.raises(Exception):
+                op._spmv(F)
+            continue
         y = op._spmv(F)[0].cpu().numpy()
         assert op._plan is not None
         assert np.all(np.abs(y - ref) <= bound), m
@@ -427,6 +433,17 @@ def test_spmv_mappings_agree(RBF, gpu, fields):
     op.SPMV_MAPPING = 2
     with pytest.raises(Exception):
         op._spmv(F, asynchronous=True)
+    if fields < 8:
+        for kk in (21, 30):
+            opk = RBF.laplacian(pts[:30_000], basis=RBF.PHS(3, poly_deg=2), k=kk, device=gpu)
+            Wk = opk.weights()
+            Fk = Fh[:30_000]
+            refk = Wk @ Fk
+            boundk = 1e-13 * (abs(Wk) @ np.abs(Fk))
+            for m in (1, 3):
+                opk.SPMV_MAPPING = m
+                y = opk._spmv(torch.as_tensor(Fk, device=gpu))[0].cpu().numpy()
+                assert np.all(np.abs(y - refk) <= boundk), (kk, m)
 
     n = 20_000
     pv = rng.random((n, 2))
@@ -444,6 +461,9 @@ def test_spmv_mappings_agree(RBF, gpu, fields):
         opv._max_row = max_row
         y = opv._spmv(F)[0].cpu().numpy()
         assert np.all(np.abs(y - ref) <= bound), (m, max_row)
+    opv.SPMV_MAPPING = 3  # rows of varying length: refused
+    with pytest.raises(Exception):
+        opv._spmv(F)
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
