@@ -382,6 +382,10 @@ __device__ __forceinline__ double term_scale(const TermDev& t, double hinv)
 // per-stencil tables in LDS
 template <typename T, int NMAX, int DIM, int UW>
 struct StencilLds {
+    // monomial values at the stencil points, pm[pi * M + a] (two stencils per wavefront:
+    // k * M <= 16 * 16 when k + M <= 32)
+    static constexpr int kPm = NMAX == 32 ? 256 : 1;
+    double pm[kPm];
     double xi[NMAX][DIM];  // scaled offsets of the stencil points
     double ba[NMAX], bb[NMAX], bn[NMAX][DIM];
     T prow[2][2][UW];  // candidate / pivot rows: two slots, double-buffered by step parity
@@ -633,6 +637,13 @@ k_build(const BuildDev P)
             }
         }
     }
+    // monomials of the own point, once: the assembly reads them for the polynomial
+    // columns of this row and the polynomial rows of this column
+    constexpr bool PM_TABLE = NMAX == 32;
+    if constexpr (PM_TABLE) {
+        if (is_pt)
+            for (int a = 0; a < M; ++a) S.pm[pi * M + a] = mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi);
+    }
     const int n = alive ? M + kk : 0;
     int n_wave = n;
 #pragma unroll
@@ -658,7 +669,8 @@ k_build(const BuildDev P)
         if (j < M) {
             // point row: monomial j at the own point (Neumann/Robin rows: alpha P + beta dP/dn)
             if (is_pt) {
-                v = mono_op<DIM>(s_mono[j], RBFX_OP_VALUE, 0, 0, xi);
+                if constexpr (PM_TABLE) v = S.pm[pi * M + j];
+                else v = mono_op<DIM>(s_mono[j], RBFX_OP_VALUE, 0, 0, xi);
                 if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[j], bn, xi);
             }
         } else {
@@ -670,7 +682,8 @@ k_build(const BuildDev P)
                 const int tj = S.bt[jj];
                 if (is_poly) {
                     // polynomial row li: monomial li at point jj
-                    v = mono_op<DIM>(s_mono[li], RBFX_OP_VALUE, 0, 0, xj);
+                    if constexpr (PM_TABLE) v = S.pm[jj * M + li];
+                    else v = mono_op<DIM>(s_mono[li], RBFX_OP_VALUE, 0, 0, xj);
                     if (hermite && tj == 2) {
                         double nj[DIM];
 #pragma unroll

@@ -218,6 +218,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
     B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
     double wt[R];
     double ptw[MM], G[MM][MM];
+    double pr[R][MM];  // the monomials at this lane's rows, evaluated once
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int a = 0; a < MM; ++a) pr[r][a] = mono2(s_mono[a], dx[r], dy[r]);
 #pragma unroll
     for (int a = 0; a < MM; ++a) {
         ptw[a] = 0.0;
@@ -231,14 +236,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         if (i < K) {
             // the network predicts w~ r1^2 (r1: the canonical reference distance)
             wt[r] = ((double)s_out[sl][i] * P.out_std[i] + P.out_mean[i]) / (r1 * r1);
-            double pr[MM];
-#pragma unroll
-            for (int a = 0; a < MM; ++a) pr[a] = mono2(s_mono[a], dx[r], dy[r]);
 #pragma unroll
             for (int a = 0; a < MM; ++a) {
-                ptw[a] += pr[a] * wt[r];
+                ptw[a] += pr[r][a] * wt[r];
 #pragma unroll
-                for (int b = 0; b <= a; ++b) G[a][b] += pr[a] * pr[b];
+                for (int b = 0; b <= a; ++b) G[a][b] += pr[r][a] * pr[r][b];
             }
         }
     }
@@ -300,7 +302,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         const int i = q + 4 * r;
         double corr = 0.0;
 #pragma unroll
-        for (int a = 0; a < MM; ++a) corr += mono2(s_mono[a], dx[r], dy[r]) * rhs[a];
+        for (int a = 0; a < MM; ++a) corr += pr[r][a] * rhs[a];
         wt[r] += corr;
         if (i < K) s_w[sl][i] = wt[r];
     }
@@ -324,14 +326,23 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
             }
             lmax = fmax(lmax, fabs(lphi));
             double acc = 0.0;
+            if (B.kind == RBFX_PHS3) {  // uniform: the surrogate's own basis, without the basis switch
 #pragma unroll 4
-            for (int j = 0; j < K; ++j) {
-                const double ex = dx[r] - s_xi[sl][j][0], ey = dy[r] - s_xi[sl][j][1];
-                acc += phi_r2(B, ex * ex + ey * ey) * s_w[sl][j];
+                for (int j = 0; j < K; ++j) {
+                    const double ex = dx[r] - s_xi[sl][j][0], ey = dy[r] - s_xi[sl][j][1];
+                    const double q2 = ex * ex + ey * ey;
+                    acc += q2 * sqrt(q2) * s_w[sl][j];
+                }
+            } else {
+#pragma unroll 4
+                for (int j = 0; j < K; ++j) {
+                    const double ex = dx[r] - s_xi[sl][j][0], ey = dy[r] - s_xi[sl][j][1];
+                    acc += phi_r2(B, ex * ex + ey * ey) * s_w[sl][j];
+                }
             }
             sres[r] = lphi - acc;
 #pragma unroll
-            for (int a = 0; a < MM; ++a) pts[a] += mono2(s_mono[a], dx[r], dy[r]) * sres[r];
+            for (int a = 0; a < MM; ++a) pts[a] += pr[r][a] * sres[r];
         }
     }
 #pragma unroll
@@ -343,7 +354,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_pe
         if (q + 4 * r < K) {
             double v = sres[r];
 #pragma unroll
-            for (int a = 0; a < MM; ++a) v -= mono2(s_mono[a], dx[r], dy[r]) * pts[a];
+            for (int a = 0; a < MM; ++a) v -= pr[r][a] * pts[a];
             rmax = fmax(rmax, fabs(v));
         }
     }

@@ -77,17 +77,25 @@ hipError_t launch_adam(int64_t n, float* p, const float* g, float* m, float* v, 
 hipError_t launch_params_to_bf16(const float* p, __bf16* wb, hipStream_t s);
 hipError_t launch_fill_f32(float* p, int64_t n, float v, hipStream_t s);
 
+// tanh(u) = 1 - 2 / (exp(2u) + 1): one hardware exp and one reciprocal. exp(2u)
+// overflows to +inf for large u (giving 1) and underflows to 0 for very negative u
+// (giving -1); near u = 0 the absolute error is a few float ulps, far below the BF16
+// rounding the activations then take.
+__device__ __forceinline__ float tanh_exp(float u)
+{
+    return 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * u) + 1.0f);
+}
 __device__ __forceinline__ float gelu_tanh(float z)
 {
     const float c = 0.7978845608028654f;  // sqrt(2/pi)
-    const float t = tanhf(c * (z + 0.044715f * z * z * z));
+    const float t = tanh_exp(c * (z + 0.044715f * z * z * z));
     return 0.5f * z * (1.0f + t);
 }
 __device__ __forceinline__ float gelu_tanh_grad(float z)
 {
     const float c = 0.7978845608028654f;
     const float u = c * (z + 0.044715f * z * z * z);
-    const float t = tanhf(u);
+    const float t = tanh_exp(u);
     return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * c * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
