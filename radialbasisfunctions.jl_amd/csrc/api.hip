@@ -65,6 +65,8 @@ struct DeviceState {
     int32_t* bucket_count = nullptr;  // [3] list lengths, [3..5] tuning sample sizes
     hipEvent_t tev[2] = {nullptr, nullptr};
     std::map<int, int> tuned;  // (dim, columns, precision, bucket) -> kernel
+    double* packed = nullptr;  // per-position stencil offsets of a register-kernel launch
+    size_t cap_packed = 0;
 };
 
 std::mutex g_mu;
@@ -321,6 +323,31 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, Device
     return RBFX_OK;
 }
 
+// Offsets of the stencils of a launch of up to 32 rows with fixed k and no Hermite rows,
+// packed by work position (launch_pack_offsets), or nullptr when the launch gathers
+// them itself (other shapes, or an asynchronous build whose buffer is too small: a
+// blocking build of the same size sizes it).
+static const double* pack_offsets(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceState& ds, int nmax,
+                                  int64_t n_work, bool async, hipStream_t st, int* rc)
+{
+    *rc = RBFX_OK;
+    if (nmax > 32 || a->adj_ptr || P.out_ptr || P.is_boundary || P.k < 1 || n_work <= 0) return nullptr;
+    const size_t need = sizeof(double) * (size_t)n_work * (size_t)P.k * (size_t)a->dim;
+    if (!async) {
+        if (grow((void**)&ds.packed, ds.cap_packed, need) != hipSuccess) {
+            *rc = set_err(RBFX_EDEVICE, "rbfx_build_weights: device allocation of %zu bytes failed", need);
+            return nullptr;
+        }
+    } else if (ds.cap_packed < need) {
+        return nullptr;
+    }
+    if (rbfx::launch_pack_offsets(P, a->dim, n_work, ds.packed, st) != hipSuccess) {
+        *rc = set_err(RBFX_EDEVICE, "rbfx_build_weights: pack launch failed");
+        return nullptr;
+    }
+    return ds.packed;
+}
+
 // Main solve of all stencils (plain FP64, or FP32 + refinement). Stencils of one size
 // run in one launch. Uneven stencils (ragged k, per-stencil degree) are classified into
 // size buckets in processing order, and each bucket runs on the kernel chosen for it.
@@ -337,12 +364,16 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceSt
     if (!uneven) {
         const int b = nmax <= 32 ? 0 : (nmax <= 64 ? 1 : 2);
         int kid = K_REG32;
-        const int rc = pick_kernel(a, P, ds, b, fp32, nmax, a->n_eval, nullptr, async, st, &kid);
+        rbfx::BuildDev Q = P;
+        int prc = RBFX_OK;
+        Q.packed = pack_offsets(a, P, ds, nmax, a->n_eval, async, st, &prc);
+        if (prc != RBFX_OK) return prc;
+        const int rc = pick_kernel(a, Q, ds, b, fp32, nmax, a->n_eval, nullptr, async, st, &kid);
         if (rc != RBFX_OK) return rc;
         g_bucket_n[b] = a->n_eval;
         g_bucket_kernel[b] = kid;
         if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
-        HIPCHK(launch_kernel_id(kid, P, dim, fp32, nmax, a->n_eval, st));
+        HIPCHK(launch_kernel_id(kid, Q, dim, fp32, nmax, a->n_eval, st));
         return RBFX_OK;
     }
     const size_t need = sizeof(int32_t) * 3 * (size_t)a->n_eval;
@@ -598,7 +629,11 @@ static int build_impl(const rbfx_build_args* a, bool async)
             if (!async && !ds.bucket_count) HIPCHK(hipMalloc((void**)&ds.bucket_count, 6 * sizeof(int32_t)));
             const int b = nmax <= 32 ? 0 : 1;
             int kid = b == 0 ? K_REG32 : K_REG64;
-            const int rc = pick_kernel(a, P, ds, b, true, nmax, n_solve, ds.bucket_count ? ds.bucket_count + 3 : nullptr,
+            rbfx::BuildDev Q = P;  // (P itself stays unpacked for the FP64 re-solve below)
+            int prc = RBFX_OK;
+            Q.packed = pack_offsets(a, P, ds, nmax, async ? a->n_eval : n_solve, async, st, &prc);
+            if (prc != RBFX_OK) return prc;
+            const int rc = pick_kernel(a, Q, ds, b, true, nmax, n_solve, ds.bucket_count ? ds.bucket_count + 3 : nullptr,
                                        async, st, &kid);
             if (rc != RBFX_OK) return rc;
             hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);  // drop tuning failures
@@ -608,7 +643,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
             }
             g_bucket_n[b] = async ? -1 : n_solve;
             g_bucket_kernel[b] = kid;
-            if (n_solve > 0) HIPCHK(launch_kernel_id(kid, P, a->dim, true, nmax, n_solve, st));
+            if (n_solve > 0) HIPCHK(launch_kernel_id(kid, Q, a->dim, true, nmax, n_solve, st));
         } else {
             const int rc = solve_all(a, P, ds, nmax, true, async, st);
             if (rc != RBFX_OK) return rc;

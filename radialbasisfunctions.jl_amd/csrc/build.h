@@ -70,6 +70,10 @@ struct BuildDev {
     double accept_tol;
     int ok_status;  // status written for a stencil solved by this launch
     int64_t n_blocks;  // workgroups with work (set by the launcher; the grid is padded)
+    // optional: the offsets x_j - x_e of each work position's k points, [position][k][dim],
+    // written by launch_pack_offsets for this launch's work order (fixed k, no Hermite
+    // rows); the column indices are then already written too
+    const double* packed;
 };
 
 int fill_mono_table(int dim, int deg, int (*e)[3]);
@@ -100,5 +104,9 @@ hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n
 bool wide_fits(int rows_cap, int n_cols);
 
 hipError_t launch_build(const BuildDev& P, int dim, int nmax, bool fp32, int64_t n_work, hipStream_t s);
+// the offsets and column indices of every work position of P (its list, or order, or
+// identity), for a build launch with P.packed = out; n_work bounds the positions (the
+// list's device length is read by the kernel)
+hipError_t launch_pack_offsets(const BuildDev& P, int dim, int64_t n_work, double* out, hipStream_t s);
 
 }  // namespace rbfx
