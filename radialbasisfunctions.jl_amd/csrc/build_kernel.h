@@ -13,14 +13,10 @@
 // into registers: lane li of a group owns row li, r[j] holds A[li][j], and the
 // operator columns ride along as register columns r[NMAX + c] (augmented system).
 //
-// Registers are only ever indexed statically. Where an index is uniform over a group
-// (the assembly column, a queried column, a pivot), the register is selected through
-// a switch on a scalar value, which the compiler lowers to scalar branches; a wave
-// holding two groups runs the switch once per group. (Rows held in 1024-bit vector
-// registers and indexed with s_set_gpr_idx_on were measured 5% faster on the 26-row
-// build, but repeated identical builds of 1-D test stencils then differed from run to
-// run in single stencils, up to wrong or singular results, while these switch rows
-// were bit-identical in every repetition; they are not used.)
+// The matrix part of a row sits in 1024-bit vector registers. Where an index is
+// uniform over a group (the assembly column, a queried column, a pivot), the entry is
+// read or written by indexed register moves (s_set_gpr_idx_on) with a scalar index; a
+// wave holding two groups does this once per group and selects by lane.
 //
 // Factorisation: symmetric-indefinite LDL^T with rook pivoting (the pivot tests of
 // Ashcraft, Grimes & Lewis 1998, as LAPACK ?sytrf_rook uses). Rows never move between
@@ -40,6 +36,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <type_traits>
 
 #include "build.h"
 #include "device_math.h"
@@ -143,84 +141,235 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long m)
     return (m >> (g * NMAX)) & ((1ull << NMAX) - 1ull);
 }
 
-// ------------------------------------------------ register selection by scalar index
-// r[j] for a wave-uniform j in [0, 64): a switch over a scalar value (scalar branches)
+// ------------------------------------------------ register rows
+// A lane's row: NMAX matrix entries held in 1024-bit vector registers (16 doubles or
+// 32 floats each), then RC right-hand-side columns in plain registers. Static indices
+// (unrolled loops) address single registers through get/set. An entry at an index that
+// is uniform over a group (the assembly column, a queried column, a pivot) is an indexed
+// register move (s_set_gpr_idx_on + v_mov) in each vector and a select of the vector,
+// one fixed branch-free sequence per group index (group_row_at, RBFX_ROW_SET,
+// RBFX_GROUP_ROW_PUT). Variable-index reads receive the vectors by value (RBFX_ROWV):
+// a read through a reference is compiled as a load at a variable offset, which keeps
+// the row in scratch memory; variable-index writes are macros in the kernel body for
+// the same reason. (ArrRow, plain registers selected by a switch on the index, is kept
+// for reference; the kernels use VecRow.)
+
 #define RBFX_CASES(X)                                                                                 \
     X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) \
     X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32) X(33)   \
     X(34) X(35) X(36) X(37) X(38) X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49)   \
     X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58) X(59) X(60) X(61) X(62) X(63)
 
-template <int N, typename T>
-__device__ __forceinline__ T reg_get(const T (&r)[N], int j)
-{
-    T v = (T)0;
-    switch (j) {
-#define RBFX_GET(q)                    \
-    case q:                            \
-        if constexpr (q < N) v = r[q]; \
+template <typename T, int NMAX, int RC>
+struct ArrRow {
+    static constexpr bool kVector = false;
+    static constexpr int N = NMAX + RC;
+    typedef T elem;
+    T r[N];
+
+    __device__ __forceinline__ void zero()
+    {
+#pragma unroll
+        for (int j = 0; j < N; ++j) r[j] = (T)0;
+    }
+    __device__ __forceinline__ T get(int j) const { return r[j]; }
+    __device__ __forceinline__ void set(int j, T x) { r[j] = x; }
+    __device__ __forceinline__ const ArrRow* a0() const { return this; }
+    __device__ __forceinline__ int a1() const { return 0; }
+    __device__ __forceinline__ int a2() const { return 0; }
+    __device__ __forceinline__ int a3() const { return 0; }
+
+    // entry j for a wave-uniform j in [0, NMAX)
+    static __device__ __forceinline__ T at(const ArrRow* p, int j)
+    {
+        T v = (T)0;
+        switch (j) {
+#define RBFX_GET(q)                         \
+    case q:                                 \
+        if constexpr (q < NMAX) v = p->r[q]; \
         break;
-        RBFX_CASES(RBFX_GET)
+            RBFX_CASES(RBFX_GET)
 #undef RBFX_GET
-    default: break;
+        default: break;
+        }
+        return v;
     }
-    return v;
-}
-
-template <int N, typename T>
-__device__ __forceinline__ void reg_set(T (&r)[N], int j, T v)
-{
-    switch (j) {
-#define RBFX_SET(q)                    \
-    case q:                            \
-        if constexpr (q < N) r[q] = v; \
+    template <int GN>
+    static __device__ __forceinline__ T group_at(const ArrRow* p, int, int, int, int idx)
+    {
+        if constexpr (GN == 64) {
+            return at(p, __builtin_amdgcn_readfirstlane(idx));
+        } else {
+            const int a = rdlane(idx, 0), b = rdlane(idx, 32);
+            const T va = at(p, a), vb = at(p, b);
+            return lane_id() >= 32 ? vb : va;
+        }
+    }
+    // entry j (wave-uniform, in [0, NMAX)) = x
+    __device__ __forceinline__ void put(int j, T x)
+    {
+        switch (j) {
+#define RBFX_SET(q)                          \
+    case q:                                  \
+        if constexpr (q < NMAX) r[q] = x;    \
         break;
-        RBFX_CASES(RBFX_SET)
+            RBFX_CASES(RBFX_SET)
 #undef RBFX_SET
-    default: break;
+        default: break;
+        }
     }
-}
-
-// r[idx] where idx is uniform over each group of NMAX lanes
-template <int NMAX, int N, typename T>
-__device__ __forceinline__ T group_reg_get(const T (&r)[N], int idx)
-{
-    if constexpr (NMAX == 64) {
-        return reg_get<N, T>(r, __builtin_amdgcn_readfirstlane(idx));
-    } else {
-        const int a = rdlane(idx, 0), b = rdlane(idx, 32);
-        const T va = reg_get<N, T>(r, a), vb = reg_get<N, T>(r, b);
-        return lane_id() >= 32 ? vb : va;
-    }
-}
-
-// r[j] = v on the lanes where pred holds (j wave-uniform; a select, no lane divergence)
-template <int N, typename T>
-__device__ __forceinline__ void reg_set_if(T (&r)[N], int j, T v, bool pred)
-{
-    switch (j) {
-#define RBFX_SETIF(q)                                 \
-    case q:                                           \
-        if constexpr (q < N) r[q] = pred ? v : r[q]; \
+    // entry j (wave-uniform, in [0, NMAX)) = x on the lanes where pred holds
+    __device__ __forceinline__ void put_if(int j, T x, bool pred)
+    {
+        switch (j) {
+#define RBFX_SETIF(q)                                    \
+    case q:                                              \
+        if constexpr (q < NMAX) r[q] = pred ? x : r[q];  \
         break;
-        RBFX_CASES(RBFX_SETIF)
+            RBFX_CASES(RBFX_SETIF)
 #undef RBFX_SETIF
-    default: break;
+        default: break;
+        }
     }
+};
+
+template <typename T, int NMAX, int RC>
+struct VecRow {
+    static constexpr bool kVector = true;
+    static constexpr int C = 128 / (int)sizeof(T);
+    static constexpr int NC = NMAX / C;
+    static_assert(NMAX % C == 0 && NC >= 1 && NC <= 4, "the matrix part fills one to four vector registers");
+    typedef T elem;
+    typedef T vec __attribute__((ext_vector_type(C)));
+    // separate members (not an array), so that every access is a whole-vector access
+    // of one variable and the row never leaves registers
+    vec v0, v1, v2, v3;
+    T t[RC > 0 ? RC : 1];
+
+    __device__ __forceinline__ void zero()
+    {
+        v0 = (vec)(T)0;
+        if constexpr (NC > 1) v1 = (vec)(T)0;
+        if constexpr (NC > 2) v2 = (vec)(T)0;
+        if constexpr (NC > 3) v3 = (vec)(T)0;
+#pragma unroll
+        for (int c = 0; c < (RC > 0 ? RC : 1); ++c) t[c] = (T)0;
+    }
+    // entry q of vector c
+    static __device__ __forceinline__ T vget(vec v0, vec v1, vec v2, vec v3, unsigned c, unsigned q)
+    {
+        T x = v0[q];
+        if constexpr (NC > 1) {
+            const T y = v1[q];
+            x = c == 1u ? y : x;
+        }
+        if constexpr (NC > 2) {
+            const T y = v2[q];
+            x = c == 2u ? y : x;
+        }
+        if constexpr (NC > 3) {
+            const T y = v3[q];
+            x = c == 3u ? y : x;
+        }
+        return x;
+    }
+    // compile-time index j in [0, NMAX + RC) (after unrolling). The right-hand-side
+    // registers are only ever addressed with constant subscripts, so that no access
+    // with a variable offset into the row exists before the loops are unrolled.
+    __device__ __forceinline__ T get(int j) const
+    {
+        if (j < NMAX) return vget(v0, v1, v2, v3, (unsigned)j / C, (unsigned)j % C);
+        T x = (T)0;
+#pragma unroll
+        for (int c = 0; c < RC; ++c) x = (j - NMAX == c) ? t[c] : x;
+        return x;
+    }
+    __device__ __forceinline__ void set(int j, T x)
+    {
+        if (j < NMAX) {
+            const unsigned c = (unsigned)j / C, q = (unsigned)j % C;
+            v0[q] = c == 0u ? x : v0[q];
+            if constexpr (NC > 1) v1[q] = c == 1u ? x : v1[q];
+            if constexpr (NC > 2) v2[q] = c == 2u ? x : v2[q];
+            if constexpr (NC > 3) v3[q] = c == 3u ? x : v3[q];
+        } else {
+#pragma unroll
+            for (int c = 0; c < RC; ++c) t[c] = (j - NMAX == c) ? x : t[c];
+        }
+    }
+    __device__ __forceinline__ vec a0() const { return v0; }
+    __device__ __forceinline__ vec a1() const { return v1; }
+    __device__ __forceinline__ vec a2() const { return v2; }
+    __device__ __forceinline__ vec a3() const { return v3; }
+
+    template <int GN>
+    static __device__ __forceinline__ T group_at(vec v0, vec v1, vec v2, vec v3, int idx)
+    {
+        if constexpr (GN == 64) {
+            const unsigned j = (unsigned)__builtin_amdgcn_readfirstlane(idx);
+            return vget(v0, v1, v2, v3, j / C, j & (C - 1));
+        } else {
+            const unsigned a = (unsigned)rdlane(idx, 0), b = (unsigned)rdlane(idx, 32);
+            const T va = vget(v0, v1, v2, v3, a / C, a & (C - 1));
+            const T vb = vget(v0, v1, v2, v3, b / C, b & (C - 1));
+            return lane_id() >= 32 ? vb : va;
+        }
+    }
+};
+
+// arguments of a variable-index read of a row
+#define RBFX_ROWV(row) (row).a0(), (row).a1(), (row).a2(), (row).a3()
+
+// row entry idx, idx uniform over each group of NMAX lanes
+template <int NMAX, typename Row, typename A0, typename A1, typename A2, typename A3>
+__device__ __forceinline__ typename Row::elem group_row_at(A0 a0, A1 a1, A2 a2, A3 a3, int idx)
+{
+    return Row::template group_at<NMAX>(a0, a1, a2, a3, idx);
 }
 
-// r[idx] = v on the lanes where pred holds; idx uniform over each group
-template <int NMAX, int N, typename T>
-__device__ __forceinline__ void group_reg_set(T (&r)[N], int idx, T v, bool pred)
-{
-    if constexpr (NMAX == 64) {
-        reg_set_if<N, T>(r, __builtin_amdgcn_readfirstlane(idx), v, pred);
-    } else {
-        const int a = rdlane(idx, 0), b = rdlane(idx, 32);
-        reg_set_if<N, T>(r, a, v, pred && lane_id() < 32);
-        reg_set_if<N, T>(r, b, v, pred && lane_id() >= 32);
-    }
-}
+// row entry j (wave-uniform, in [0, NMAX)) = x on the lanes where pred holds. A macro,
+// not a function, for a VecRow: a variable-index write compiled inside a callee becomes
+// a store through a pointer at a variable offset, which after inlining pins the row in
+// scratch memory; written in the kernel body it stays an indexed register move.
+#define RBFX_ROW_PUT(row, j, x, pred)                                                          \
+    do {                                                                                       \
+        using Row_ = std::remove_reference_t<decltype(row)>;                                   \
+        const unsigned j_ = (unsigned)(j);                                                     \
+        const auto x_ = (x);                                                                   \
+        const bool p_ = (pred);                                                                \
+        if constexpr (Row_::kVector) {                                                         \
+            const unsigned q_ = j_ & (unsigned)(Row_::C - 1), c_ = j_ / (unsigned)Row_::C;     \
+            (row).v0[q_] = (p_ && c_ == 0u) ? x_ : (row).v0[q_];                               \
+            if constexpr (Row_::NC > 1) (row).v1[q_] = (p_ && c_ == 1u) ? x_ : (row).v1[q_];   \
+            if constexpr (Row_::NC > 2) (row).v2[q_] = (p_ && c_ == 2u) ? x_ : (row).v2[q_];   \
+            if constexpr (Row_::NC > 3) (row).v3[q_] = (p_ && c_ == 3u) ? x_ : (row).v3[q_];   \
+        } else {                                                                               \
+            (row).put_if((int)j_, x_, p_);                                                     \
+        }                                                                                      \
+    } while (0)
+
+// row entry j (wave-uniform, in [0, NMAX)) = x on every lane
+#define RBFX_ROW_SET(row, j, x)                                                                \
+    do {                                                                                       \
+        using Row_ = std::remove_reference_t<decltype(row)>;                                   \
+        if constexpr (Row_::kVector) {                                                         \
+            RBFX_ROW_PUT(row, j, x, true);                                                     \
+        } else {                                                                               \
+            (row).put((int)(j), (x));                                                          \
+        }                                                                                      \
+    } while (0)
+
+// row[idx] = x on the lanes where pred holds; idx uniform over each group of NMAX lanes
+#define RBFX_GROUP_ROW_PUT(NMAX_, row, idx, x, pred)                                           \
+    do {                                                                                       \
+        if constexpr ((NMAX_) == 64) {                                                         \
+            RBFX_ROW_PUT(row, __builtin_amdgcn_readfirstlane(idx), x, pred);                   \
+        } else {                                                                               \
+            const int ga_ = rdlane((int)(idx), 0), gb_ = rdlane((int)(idx), 32);               \
+            RBFX_ROW_PUT(row, ga_, x, (pred) && lane_id() < 32);                               \
+            RBFX_ROW_PUT(row, gb_, x, (pred) && lane_id() >= 32);                              \
+        }                                                                                      \
+    } while (0)
 
 // term scale h^-order in the radius-scaled frame
 __device__ __forceinline__ double term_scale(const TermDev& t, double hinv)
@@ -246,8 +395,9 @@ struct StencilLds {
 // Solves U x = y (RC right-hand sides at once) for this lane's group. Row p of U is
 // the frozen register row of the lane that pivoted at slot s (s_piv[s] = p), with the
 // block structure in s_blk (1: 1x1, 2: first slot of a 2x2 block, 0: its second slot).
-template <int NMAX, int RC, typename T, int N>
-__device__ __forceinline__ void back_substitute(const T (&r)[N], const int* s_piv, const int* s_blk, int nslots,
+template <int NMAX, int RC, typename Row, typename A0, typename A1, typename A2, typename A3,
+          typename T = typename Row::elem>
+__device__ __forceinline__ void back_substitute(A0 r0, A1 r1, A2 r2, A3 r3, const int* s_piv, const int* s_blk, int nslots,
                                                 int nslots_wave, int myslot, T diag, int n_cols, const T (&y)[RC],
                                                 T (&x)[RC])
 {
@@ -267,11 +417,11 @@ __device__ __forceinline__ void back_substitute(const T (&r)[N], const int* s_pi
         const int q = two ? s_piv[s] : 0;
         // the second row of a 2x2 block only when one of the wave's groups has one here
         const bool two_any = __ballot(two) != 0ull;
-        const T up = group_reg_get<NMAX>(r, p);
+        const T up = group_row_at<NMAX, Row>(r0, r1, r2, r3, p);
         T uq = (T)0, dp = (T)1, dq = (T)1, bq = (T)0, idet = (T)1;
         if (two_any) {
             dp = group_bcast<NMAX>(diag, p);
-            uq = group_reg_get<NMAX>(r, q);
+            uq = group_row_at<NMAX, Row>(r0, r1, r2, r3, q);
             dq = group_bcast<NMAX>(diag, q);
             bq = group_bcast<NMAX>(uq, p);  // U[p][q]
             idet = (T)1 / (dp * dq - bq * bq);
@@ -295,8 +445,9 @@ __device__ __forceinline__ void back_substitute(const T (&r)[N], const int* s_pi
 }
 
 // forward replay of the multipliers (indexed by pivot row) on a new right-hand side
-template <int NMAX, typename T, int N>
-__device__ __forceinline__ T forward_replay(const T (&Lm)[N], const int* s_piv, const int* s_blk, int nslots,
+template <int NMAX, typename Row, typename A0, typename A1, typename A2, typename A3,
+          typename T = typename Row::elem>
+__device__ __forceinline__ T forward_replay(A0 m0, A1 m1, A2 m2, A3 m3, const int* s_piv, const int* s_blk, int nslots,
                                             int nslots_wave, int myslot, T y)
 {
     for (int s = 0; s < nslots_wave; ++s) {
@@ -305,7 +456,8 @@ __device__ __forceinline__ T forward_replay(const T (&Lm)[N], const int* s_piv, 
         const int q = blk == 2 ? s_piv[s + 1] : 0;
         const bool two_any = __ballot(blk == 2) != 0ull;
         const T y0 = group_bcast<NMAX>(y, p), y1 = two_any ? group_bcast<NMAX>(y, q) : (T)0;
-        const T l0 = group_reg_get<NMAX>(Lm, p), l1 = two_any ? group_reg_get<NMAX>(Lm, q) : (T)0;
+        const T l0 = group_row_at<NMAX, Row>(m0, m1, m2, m3, p);
+        const T l1 = two_any ? group_row_at<NMAX, Row>(m0, m1, m2, m3, q) : (T)0;
         if (blk == 1 && myslot > s) y -= l0 * y0;
         if (blk == 2 && myslot > s + 1) y -= l0 * y0 + l1 * y1;
     }
@@ -315,23 +467,23 @@ __device__ __forceinline__ T forward_replay(const T (&Lm)[N], const int* s_piv, 
 // r += a * u over a broadcast LDS row, in chunks of 8 columns: the scheduling
 // barriers keep at most one chunk of the row in flight, so the register row is
 // never spilled to make room for prefetched operands
-template <int NMAX, int RC, typename T, int N>
-__device__ __forceinline__ void axpy_row(T (&r)[N], T a, const T* u, int n_cols)
+template <int NMAX, int RC, typename Row, typename T>
+__device__ __forceinline__ void axpy_row(Row& r, T a, const T* u, int n_cols)
 {
 #pragma unroll
     for (int j0 = 0; j0 < NMAX; j0 += 8) {
 #pragma unroll
-        for (int j = j0; j < j0 + 8; ++j) r[j] = fma(a, u[j], r[j]);
+        for (int j = j0; j < j0 + 8; ++j) r.set(j, fma(a, u[j], r.get(j)));
         __builtin_amdgcn_sched_barrier(0);
     }
 #pragma unroll
     for (int c = 0; c < RC; ++c)
-        if (c < n_cols) r[NMAX + c] = fma(a, u[NMAX + c], r[NMAX + c]);
+        if (c < n_cols) r.set(NMAX + c, fma(a, u[NMAX + c], r.get(NMAX + c)));
 }
 
 // waves per SIMD the register allocation targets (FP64, two stencils per wavefront,
 // up to 4 right-hand sides). Two waves fit the kernel in registers. Three waves (168
-// VGPRs, 52-108 bytes per lane spilled to scratch) measured 27% faster on the 4M-stencil
+// VGPRs, 52-108 bytes per lane spilled to scratch) measured faster on the 4M-stencil
 // 2-D build, but repeated identical builds of 1-D stencils (11-row systems, short-lived
 // wavefronts) then differed from run to run in single stencils, up to wrong and singular
 // results, on the wavefronts that were the third on their compute unit; the spill-free
@@ -515,13 +667,13 @@ k_build(const BuildDev P)
     B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
 
     // ---- assembly (FP64): column j of every row in step j (j uniform over the wave)
-    T r[UW];
-    double a64[REFINE ? NMAX : 1];
+    using RowT = VecRow<T, NMAX, RC>;
+    using MulRow = VecRow<T, NMAX, 0>;
+    RowT r;
+    VecRow<double, NMAX, 0> a64;  // FP64 copy of the matrix row (FP32 mode only)
     double diag64 = 0.0;
-#pragma unroll
-    for (int j = 0; j < UW; ++j) r[j] = (T)0;
-#pragma unroll
-    for (int j = 0; j < (REFINE ? NMAX : 1); ++j) a64[j] = 0.0;
+    r.zero();
+    if constexpr (REFINE) a64.zero();
     const bool is_poly = alive && li < M;
     for (int j = 0; j < n_wave; ++j) {
         double v = 0.0;
@@ -572,8 +724,8 @@ k_build(const BuildDev P)
                 }
             }
         }
-        reg_set<UW, T>(r, j, (T)v);
-        if constexpr (REFINE) reg_set<NMAX, double>(a64, j, v);
+        RBFX_ROW_SET(r, j, (T)v);
+        if constexpr (REFINE) RBFX_ROW_SET(a64, j, v);
     }
 
     // ---- right-hand sides (FP64), augmented as register columns NMAX..NMAX+RC-1
@@ -625,7 +777,7 @@ k_build(const BuildDev P)
         }
 #pragma unroll
         for (int c = 0; c < RC; ++c) {
-            r[NMAX + c] = (T)bcol[c];
+            r.set(NMAX + c, (T)bcol[c]);
             if constexpr (REFINE) b64[c] = bcol[c];
         }
     }
@@ -633,9 +785,8 @@ k_build(const BuildDev P)
     // ---- factorisation with rook pivoting (runtime step loop; every lane takes part)
     const T alpha = (T)0.6403882032022076;  // (1 + sqrt(17)) / 8
     T diag = (T)diag64;
-    T Lm[REFINE ? NMAX : 1];
-#pragma unroll
-    for (int j = 0; j < (REFINE ? NMAX : 1); ++j) Lm[j] = (T)0;
+    MulRow Lm;  // multipliers by pivot row (FP32 mode only)
+    if constexpr (REFINE) Lm.zero();
     bool act = alive && li < n;
     int myslot = 1 << 20;
     int slot = 0;
@@ -655,7 +806,7 @@ k_build(const BuildDev P)
             // row(s) are then already in their slots for the update.
             if (busy && li == c0) {
 #pragma unroll
-                for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
+                for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r.get(j);
             }
             __syncthreads();
             const T cv = (T)fabs(S.prow[buf][0][li]);
@@ -674,7 +825,7 @@ k_build(const BuildDev P)
                 const int si = 1 - sp;  // row imax into the slot not holding row p
                 if (need_rook && li == imax) {
 #pragma unroll
-                    for (int j = 0; j < UW; ++j) S.prow[buf][si][j] = r[j];
+                    for (int j = 0; j < UW; ++j) S.prow[buf][si][j] = r.get(j);
                 }
                 __syncthreads();
                 const T rv = need_rook ? (T)fabs(S.prow[buf][si][li]) : (T)0;
@@ -703,7 +854,7 @@ k_build(const BuildDev P)
         } else {
             // one group per wavefront: the selection index is wave-uniform, so a column
             // entry is a scalar-indexed register read and a value a single lane read
-            const T cv = (T)fabs(group_reg_get<NMAX>(r, c0));  // column c0 = row c0 by symmetry
+            const T cv = (T)fabs(group_row_at<NMAX, RowT>(RBFX_ROWV(r), c0));  // column c0 = row c0 by symmetry
             const uint32_t ck = group_max_u32<NMAX>((act && li != c0) ? argmax_key(cv, li) : 0u);
             int imax = ck ? 63 - (int)(ck & 63u) : c0;
             T colmax = ck ? group_bcast<NMAX>(cv, imax) : (T)0;
@@ -715,7 +866,7 @@ k_build(const BuildDev P)
             }
             for (int it = 0; it < 2 * NMAX; ++it) {
                 if (__ballot(need_rook) == 0ull) break;
-                const T rv = (T)fabs(group_reg_get<NMAX>(r, imax));
+                const T rv = (T)fabs(group_row_at<NMAX, RowT>(RBFX_ROWV(r), imax));
                 const uint32_t rk = group_max_u32<NMAX>((need_rook && act && li != imax) ? argmax_key(rv, li) : 0u);
                 const int ri = rk ? 63 - (int)(rk & 63u) : imax;
                 const T rowmax = rk ? group_bcast<NMAX>(rv, ri) : (T)0;
@@ -741,11 +892,11 @@ k_build(const BuildDev P)
             const int p0r = (kstep == 1) ? kp : p;
             if (busy && !failed && li == p0r) {
 #pragma unroll
-                for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r[j];
+                for (int j = 0; j < UW; ++j) S.prow[buf][0][j] = r.get(j);
             }
             if (two && busy && !failed && li == kp) {
 #pragma unroll
-                for (int j = 0; j < UW; ++j) S.prow[buf][1][j] = r[j];
+                for (int j = 0; j < UW; ++j) S.prow[buf][1][j] = r.get(j);
             }
             __syncthreads();
             sp = 0;
@@ -800,8 +951,8 @@ k_build(const BuildDev P)
             axpy_row<NMAX, RC>(r, ml1, U1, n_cols);
         }
         if constexpr (REFINE) {
-            group_reg_set<NMAX>(Lm, p0, l0, mine);
-            if (two_any) group_reg_set<NMAX>(Lm, p1, l1, mine && kstep == 2);
+            RBFX_GROUP_ROW_PUT(NMAX, Lm, p0, l0, mine);
+            if (two_any) RBFX_GROUP_ROW_PUT(NMAX, Lm, p1, l1, mine && kstep == 2);
         }
         if (go && !failed) {
             if (li == p0) {
@@ -829,8 +980,8 @@ k_build(const BuildDev P)
 
     T y[RC], x[RC];
 #pragma unroll
-    for (int c = 0; c < RC; ++c) y[c] = r[NMAX + c];
-    back_substitute<NMAX, RC, T>(r, S.piv, S.blk, nslots, nslots_wave, myslot, diag, n_cols, y, x);
+    for (int c = 0; c < RC; ++c) y[c] = r.get(NMAX + c);
+    back_substitute<NMAX, RC, RowT>(RBFX_ROWV(r), S.piv, S.blk, nslots, nslots_wave, myslot, diag, n_cols, y, x);
     if constexpr (!REFINE) {
 #pragma unroll
         for (int c = 0; c < RC; ++c) {
@@ -852,11 +1003,11 @@ k_build(const BuildDev P)
                     __syncthreads();
                     double res = b64[c];
 #pragma unroll
-                    for (int j = 0; j < NMAX; ++j) res -= a64[j] * s_x[g][j];
+                    for (int j = 0; j < NMAX; ++j) res -= a64.get(j) * s_x[g][j];
                     __syncthreads();
-                    T yy[1] = {forward_replay<NMAX, T>(Lm, S.piv, S.blk, nslots, nslots_wave, myslot, (T)res)};
+                    T yy[1] = {forward_replay<NMAX, MulRow>(RBFX_ROWV(Lm), S.piv, S.blk, nslots, nslots_wave, myslot, (T)res)};
                     T dx[1];
-                    back_substitute<NMAX, 1, T>(r, S.piv, S.blk, nslots, nslots_wave, myslot, diag, 1, yy, dx);
+                    back_substitute<NMAX, 1, RowT>(RBFX_ROWV(r), S.piv, S.blk, nslots, nslots_wave, myslot, diag, 1, yy, dx);
                     x64 += (double)dx[0];
                     const double num = group_max<NMAX>(li < n ? fabs((double)dx[0]) : 0.0);
                     const double den = group_max<NMAX>(li < n ? fabs(x64) : 0.0);
