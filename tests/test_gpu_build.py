@@ -133,6 +133,27 @@ def _basis_obj(RBF, name, eps, deg):
     return (RBF.IMQ if name == "IMQ" else RBF.Gaussian)(eps, poly_deg=deg)
 
 
+def test_repeated_builds_bit_identical(RBF, gpu):
+    """Identical builds give bit-identical weights: 12 repetitions of the 1-D PHS3 and
+    Gaussian operators (11-row systems, two stencils per wavefront, short-lived
+    wavefronts) and of a 2-D PHS3 operator. (A spilling three-waves-per-SIMD build of
+    the same kernel differed from run to run here in single stencils.)"""
+    rng = np.random.default_rng(2024)
+    cases = [(rng.random((1500, 1)), RBF.PHS(3, poly_deg=2), 8),
+             (rng.random((1500, 1)), RBF.Gaussian(562.5, poly_deg=2), 8),
+             (rng.random((20000, 2)), RBF.PHS(3, poly_deg=2), 20)]
+    for pts, basis, k in cases:
+        L = RBF.Laplacian() + 0.5 * RBF.Partial(1, 1)
+        first = None
+        for rep in range(12):
+            op = RBF.RadialBasisOperator(L, pts, basis=basis, k=k)
+            w = op.vals.clone()
+            if first is None:
+                first = w
+            else:
+                assert bool((w == first).all()), (pts.shape, rep)
+
+
 @pytest.mark.parametrize("name,kind,eps", BASES)
 @pytest.mark.parametrize("dim,deg,k", [(2, 2, 20), (3, 1, 16), (2, 0, 12), (1, 2, 8), (3, 3, 40), (2, 3, 30)])
 def test_build_parity_bases(RBF, O, gpu, name, kind, eps, dim, deg, k):
