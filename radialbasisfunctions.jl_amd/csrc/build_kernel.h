@@ -481,25 +481,23 @@ __device__ __forceinline__ void axpy_row(Row& r, T a, const T* u, int n_cols)
         if (c < n_cols) r.set(NMAX + c, fma(a, u[NMAX + c], r.get(NMAX + c)));
 }
 
-// waves per SIMD the register allocation targets (FP64, two stencils per wavefront,
-// up to 4 right-hand sides). Two waves fit the kernel in registers. Three waves (168
-// VGPRs, 52-108 bytes per lane spilled to scratch) measured faster on the 4M-stencil
-// 2-D build, but repeated identical builds of 1-D stencils (11-row systems, short-lived
-// wavefronts) then differed from run to run in single stencils, up to wrong and singular
-// results, on the wavefronts that were the third on their compute unit; the spill-free
-// two-wave kernel was bit-identical in every repetition. Two waves are used.
-#ifndef RBFX_FP64_G2_WAVES
-#define RBFX_FP64_G2_WAVES 2
-#endif
-template <typename T, int NMAX, int RC>
+// Waves per SIMD the register allocation targets. The FP64 two-stencil kernels (up to
+// 4 right-hand sides) fit 3 waves (168 VGPRs) without spilling in 1-D and 2-D; in 3-D
+// they would spill at 3 waves and run at 2. No build kernel of two stencils per
+// wavefront may spill: with a spilling three-wave kernel (52-108 bytes per lane),
+// repeated identical builds of 1-D stencils (11-row systems, short-lived wavefronts)
+// differed from run to run in single stencils, up to wrong and singular results, on
+// the wavefronts that were the third on their compute unit; the spill-free kernels are
+// bit-identical in every repetition.
+template <typename T, int NMAX, int DIM, int RC>
 struct BuildOccupancy {
-    static constexpr int waves = sizeof(T) == 4 ? 2 : (NMAX <= 32 ? (RC <= 4 ? RBFX_FP64_G2_WAVES : 2) : 2);
+    static constexpr int waves = (sizeof(T) == 8 && NMAX <= 32 && RC <= 4 && DIM <= 2) ? 3 : 2;
 };
 
 // ------------------------------------------------------------------ kernel
 template <typename T, int NMAX, int DIM, int RC>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64),
-                          amdgpu_waves_per_eu(BuildOccupancy<T, NMAX, RC>::waves))) void
+                          amdgpu_waves_per_eu(BuildOccupancy<T, NMAX, DIM, RC>::waves))) void
 k_build(const BuildDev P)
 {
     constexpr bool REFINE = sizeof(T) == 4;
@@ -676,6 +674,11 @@ k_build(const BuildDev P)
     if constexpr (REFINE) a64.zero();
     const bool is_poly = alive && li < M;
     for (int j = 0; j < n_wave; ++j) {
+        // the shape parameter re-enters the loop as an opaque value, so the constants
+        // derived from it for every basis kind are not hoisted out of the loop (they
+        // would stay live across it and spill)
+        BasisDev Bj = B;
+        asm volatile("" : "+v"(Bj.eps));
         double v = 0.0;
         if (j < M) {
             // point row: monomial j at the own point (Neumann/Robin rows: alpha P + beta dP/dn)
@@ -708,13 +711,13 @@ k_build(const BuildDev P)
                         Dj[d] = xi[d] - xj[d];
                         q2 += Dj[d] * Dj[d];
                     }
-                    v = phi_r2(B, q2);
+                    v = phi_r2(Bj, q2);
                     if (hermite && (bt == 2 || tj == 2)) {
                         double nj[DIM];
 #pragma unroll
                         for (int d = 0; d < DIM; ++d) nj[d] = S.bn[jj][d];
                         double Di, Djn, D2;
-                        phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
+                        phi_grad_hess_dirs<DIM>(Bj, Dj, q2, bn, nj, Di, Djn, D2);
                         const double aj = S.ba[jj], bj = S.bb[jj];
                         if (bt != 2) v = aj * v - bj * Djn;
                         else if (tj != 2) v = ba * v + bb * Di;
@@ -742,15 +745,33 @@ k_build(const BuildDev P)
                 De[d] = -xi[d];
                 q2 += De[d] * De[d];
             }
-            for (int t = 0; t < n_terms; ++t) {
-                const TermDev tm = s_terms[t];
-                double v = phi_term<DIM>(B, tm.kind, tm.d1, tm.d2, De, q2);
-                if (hermite && bt == 2) v = ba * v + bb * phi_term_normal<DIM>(B, tm.kind, tm.d1, De, q2, bn);
-                const double sc = term_scale(tm, hinv);
-#pragma unroll
-                for (int c = 0; c < RC; ++c)
-                    if (tm.col == c) bcol[c] += sc * v;
+            // one term loop per basis kind: with the kind a constant, only its own
+            // derivative formulas are compiled into the loop (with the kind a runtime
+            // value, the compiler hoists the loop-invariant parts of every kind's
+            // formulas out of the loop and spills them)
+#define RBFX_RHS_TERMS(BK)                                                                              \
+    {                                                                                                   \
+        BasisDev Bk;                                                                                    \
+        Bk.kind = (BK);                                                                                 \
+        Bk.eps = B.eps;                                                                                 \
+        for (int t = 0; t < n_terms; ++t) {                                                             \
+            const TermDev tm = s_terms[t];                                                              \
+            double v = phi_term<DIM>(Bk, tm.kind, tm.d1, tm.d2, De, q2);                                \
+            if (hermite && bt == 2) v = ba * v + bb * phi_term_normal<DIM>(Bk, tm.kind, tm.d1, De, q2, bn); \
+            const double sc = term_scale(tm, hinv);                                                     \
+            for (int c = 0; c < RC; ++c)                                                                \
+                if (tm.col == c) bcol[c] += sc * v;                                                     \
+        }                                                                                               \
+    }
+            switch (B.kind) {
+            case RBFX_PHS1: RBFX_RHS_TERMS(RBFX_PHS1) break;
+            case RBFX_PHS3: RBFX_RHS_TERMS(RBFX_PHS3) break;
+            case RBFX_PHS5: RBFX_RHS_TERMS(RBFX_PHS5) break;
+            case RBFX_PHS7: RBFX_RHS_TERMS(RBFX_PHS7) break;
+            case RBFX_IMQ: RBFX_RHS_TERMS(RBFX_IMQ) break;
+            default: RBFX_RHS_TERMS(RBFX_GAUSSIAN) break;
             }
+#undef RBFX_RHS_TERMS
         } else if (is_poly) {
             double z[DIM];
 #pragma unroll
