@@ -394,9 +394,10 @@ static int staged_g(int64_t L) { return L <= 20 ? 4 : (L <= 40 ? 8 : 16); }
 // partial sums of a row are added by the row's first lane.
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-template <int NV, int E, int FMAX, bool UNITX, bool PERM>
-__global__ __launch_bounds__(256) void k_spmv_vec(const SpmvDev P, int G, int64_t nb_real)
+template <int NV, int E, int FMAX, bool UNITX, bool PERM, int GS>
+__global__ __launch_bounds__(256) void k_spmv_vec(const SpmvDev P, int G_rt, int64_t nb_real)
 {
+    const int G = GS > 0 ? GS : G_rt;  // lanes per row (a constant for k = 20)
     const int64_t lb = xcd_block(nb_real);
     if (lb < 0) return;
     const int lane = threadIdx.x & 63;
@@ -513,12 +514,14 @@ static hipError_t spmv_vec_perm(const SpmvDev& P, int e, hipStream_t s)
     const int64_t rows_per_block = 4 * (64 / G);
     const int64_t nbr = (P.n_rows + rows_per_block - 1) / rows_per_block;
     const dim3 grid((unsigned)pad8(nbr));
-    if (e == 4)
-        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM>), grid, dim3(256), 0, s, P, G, nbr);
+    if (e == 4 && G == 5)
+        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM, 5>), grid, dim3(256), 0, s, P, G, nbr);
+    else if (e == 4)
+        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM, 0>), grid, dim3(256), 0, s, P, G, nbr);
     else if (e == 2)
-        hipLaunchKernelGGL((k_spmv_vec<NV, 2, FMAX, UNITX, PERM>), grid, dim3(256), 0, s, P, G, nbr);
+        hipLaunchKernelGGL((k_spmv_vec<NV, 2, FMAX, UNITX, PERM, 0>), grid, dim3(256), 0, s, P, G, nbr);
     else
-        hipLaunchKernelGGL((k_spmv_vec<NV, 1, FMAX, UNITX, PERM>), grid, dim3(256), 0, s, P, G, nbr);
+        hipLaunchKernelGGL((k_spmv_vec<NV, 1, FMAX, UNITX, PERM, 0>), grid, dim3(256), 0, s, P, G, nbr);
     return hipGetLastError();
 }
 
