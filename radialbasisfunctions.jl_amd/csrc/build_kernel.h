@@ -151,91 +151,11 @@ __device__ __forceinline__ unsigned long long group_bits(unsigned long long m)
 // RBFX_GROUP_ROW_PUT). Variable-index reads receive the vectors by value (RBFX_ROWV):
 // a read through a reference is compiled as a load at a variable offset, which keeps
 // the row in scratch memory; variable-index writes are macros in the kernel body for
-// the same reason. (ArrRow, plain registers selected by a switch on the index, is kept
-// for reference; the kernels use VecRow.)
-
-#define RBFX_CASES(X)                                                                                 \
-    X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) \
-    X(18) X(19) X(20) X(21) X(22) X(23) X(24) X(25) X(26) X(27) X(28) X(29) X(30) X(31) X(32) X(33)   \
-    X(34) X(35) X(36) X(37) X(38) X(39) X(40) X(41) X(42) X(43) X(44) X(45) X(46) X(47) X(48) X(49)   \
-    X(50) X(51) X(52) X(53) X(54) X(55) X(56) X(57) X(58) X(59) X(60) X(61) X(62) X(63)
-
-template <typename T, int NMAX, int RC>
-struct ArrRow {
-    static constexpr bool kVector = false;
-    static constexpr int N = NMAX + RC;
-    typedef T elem;
-    T r[N];
-
-    __device__ __forceinline__ void zero()
-    {
-#pragma unroll
-        for (int j = 0; j < N; ++j) r[j] = (T)0;
-    }
-    __device__ __forceinline__ T get(int j) const { return r[j]; }
-    __device__ __forceinline__ void set(int j, T x) { r[j] = x; }
-    __device__ __forceinline__ const ArrRow* a0() const { return this; }
-    __device__ __forceinline__ int a1() const { return 0; }
-    __device__ __forceinline__ int a2() const { return 0; }
-    __device__ __forceinline__ int a3() const { return 0; }
-
-    // entry j for a wave-uniform j in [0, NMAX)
-    static __device__ __forceinline__ T at(const ArrRow* p, int j)
-    {
-        T v = (T)0;
-        switch (j) {
-#define RBFX_GET(q)                         \
-    case q:                                 \
-        if constexpr (q < NMAX) v = p->r[q]; \
-        break;
-            RBFX_CASES(RBFX_GET)
-#undef RBFX_GET
-        default: break;
-        }
-        return v;
-    }
-    template <int GN>
-    static __device__ __forceinline__ T group_at(const ArrRow* p, int, int, int, int idx)
-    {
-        if constexpr (GN == 64) {
-            return at(p, __builtin_amdgcn_readfirstlane(idx));
-        } else {
-            const int a = rdlane(idx, 0), b = rdlane(idx, 32);
-            const T va = at(p, a), vb = at(p, b);
-            return lane_id() >= 32 ? vb : va;
-        }
-    }
-    // entry j (wave-uniform, in [0, NMAX)) = x
-    __device__ __forceinline__ void put(int j, T x)
-    {
-        switch (j) {
-#define RBFX_SET(q)                          \
-    case q:                                  \
-        if constexpr (q < NMAX) r[q] = x;    \
-        break;
-            RBFX_CASES(RBFX_SET)
-#undef RBFX_SET
-        default: break;
-        }
-    }
-    // entry j (wave-uniform, in [0, NMAX)) = x on the lanes where pred holds
-    __device__ __forceinline__ void put_if(int j, T x, bool pred)
-    {
-        switch (j) {
-#define RBFX_SETIF(q)                                    \
-    case q:                                              \
-        if constexpr (q < NMAX) r[q] = pred ? x : r[q];  \
-        break;
-            RBFX_CASES(RBFX_SETIF)
-#undef RBFX_SETIF
-        default: break;
-        }
-    }
-};
+// the same reason. (Plain registers selected by a switch on the index, a scalar branch
+// tree per access, measured slower: C4 FP64 73.0 against 67.5 ms at two waves.)
 
 template <typename T, int NMAX, int RC>
 struct VecRow {
-    static constexpr bool kVector = true;
     static constexpr int C = 128 / (int)sizeof(T);
     static constexpr int NC = NMAX / C;
     static_assert(NMAX % C == 0 && NC >= 1 && NC <= 4, "the matrix part fills one to four vector registers");
@@ -328,36 +248,24 @@ __device__ __forceinline__ typename Row::elem group_row_at(A0 a0, A1 a1, A2 a2, 
 }
 
 // row entry j (wave-uniform, in [0, NMAX)) = x on the lanes where pred holds. A macro,
-// not a function, for a VecRow: a variable-index write compiled inside a callee becomes
-// a store through a pointer at a variable offset, which after inlining pins the row in
-// scratch memory; written in the kernel body it stays an indexed register move.
+// not a function: a variable-index write compiled inside a callee becomes a store
+// through a pointer at a variable offset, which after inlining pins the row in scratch
+// memory; written in the kernel body it stays an indexed register move.
 #define RBFX_ROW_PUT(row, j, x, pred)                                                          \
     do {                                                                                       \
         using Row_ = std::remove_reference_t<decltype(row)>;                                   \
         const unsigned j_ = (unsigned)(j);                                                     \
         const auto x_ = (x);                                                                   \
         const bool p_ = (pred);                                                                \
-        if constexpr (Row_::kVector) {                                                         \
-            const unsigned q_ = j_ & (unsigned)(Row_::C - 1), c_ = j_ / (unsigned)Row_::C;     \
-            (row).v0[q_] = (p_ && c_ == 0u) ? x_ : (row).v0[q_];                               \
-            if constexpr (Row_::NC > 1) (row).v1[q_] = (p_ && c_ == 1u) ? x_ : (row).v1[q_];   \
-            if constexpr (Row_::NC > 2) (row).v2[q_] = (p_ && c_ == 2u) ? x_ : (row).v2[q_];   \
-            if constexpr (Row_::NC > 3) (row).v3[q_] = (p_ && c_ == 3u) ? x_ : (row).v3[q_];   \
-        } else {                                                                               \
-            (row).put_if((int)j_, x_, p_);                                                     \
-        }                                                                                      \
+        const unsigned q_ = j_ & (unsigned)(Row_::C - 1), c_ = j_ / (unsigned)Row_::C;         \
+        (row).v0[q_] = (p_ && c_ == 0u) ? x_ : (row).v0[q_];                                   \
+        if constexpr (Row_::NC > 1) (row).v1[q_] = (p_ && c_ == 1u) ? x_ : (row).v1[q_];       \
+        if constexpr (Row_::NC > 2) (row).v2[q_] = (p_ && c_ == 2u) ? x_ : (row).v2[q_];       \
+        if constexpr (Row_::NC > 3) (row).v3[q_] = (p_ && c_ == 3u) ? x_ : (row).v3[q_];       \
     } while (0)
 
 // row entry j (wave-uniform, in [0, NMAX)) = x on every lane
-#define RBFX_ROW_SET(row, j, x)                                                                \
-    do {                                                                                       \
-        using Row_ = std::remove_reference_t<decltype(row)>;                                   \
-        if constexpr (Row_::kVector) {                                                         \
-            RBFX_ROW_PUT(row, j, x, true);                                                     \
-        } else {                                                                               \
-            (row).put((int)(j), (x));                                                          \
-        }                                                                                      \
-    } while (0)
+#define RBFX_ROW_SET(row, j, x) RBFX_ROW_PUT(row, j, x, true)
 
 // row[idx] = x on the lanes where pred holds; idx uniform over each group of NMAX lanes
 #define RBFX_GROUP_ROW_PUT(NMAX_, row, idx, x, pred)                                           \
