@@ -394,39 +394,46 @@ static int staged_g(int64_t L) { return L <= 20 ? 4 : (L <= 40 ? 8 : 16); }
 // partial sums of a row are added by the row's first lane.
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 
-template <int NV, int E, int FMAX, bool UNITX, bool PERM, int GS>
+template <int NV, int E, int FMAX, bool UNITX, bool PERM, int GS, int U>
 __global__ __launch_bounds__(256) void k_spmv_vec(const SpmvDev P, int G_rt, int64_t nb_real)
 {
     const int G = GS > 0 ? GS : G_rt;  // lanes per row (a constant for k = 20)
     const int64_t lb = xcd_block(nb_real);
     if (lb < 0) return;
     const int lane = threadIdx.x & 63;
-    const int rw = 64 / G;  // rows per wavefront
+    const int rw = 64 / G;  // rows per wavefront and unit
     const int rl = lane / G, sub = lane - rl * G;
-    const int64_t r = (lb * 4 + (threadIdx.x >> 6)) * rw + rl;
-    const bool live = rl < rw && r < P.n_rows;
+    // a wavefront takes U units of rw consecutive rows: unit u holds rows r0 + u * rw + [0, rw)
+    const int64_t r0 = (lb * 4 + (threadIdx.x >> 6)) * (int64_t)(U * rw);
     const int k = P.k;
-    double acc[NV][FMAX];
+    const int64_t last = P.n_rows - 1;
+    // rows past the end (and the lanes past the wavefront's last row) read row n_rows - 1,
+    // so every load below is in bounds and issued unconditionally; they write nothing
+    int64_t r[U];
+    bool live[U];
 #pragma unroll
-    for (int v = 0; v < NV; ++v)
+    for (int u = 0; u < U; ++u) {
+        const int64_t ru = r0 + u * rw + rl;
+        live[u] = rl < rw && ru <= last;
+        r[u] = live[u] ? ru : last;
+    }
+    const int j0 = sub * E;
+    int c[U][E];
+    double w[U][NV][E];
 #pragma unroll
-        for (int f = 0; f < FMAX; ++f) acc[v][f] = 0.0;
-    if (live) {
+    for (int u = 0; u < U; ++u) {
         // PERM: the entries are stored in processing order, so their addresses do not
         // wait for the row_order read (done after the gathers are issued)
-        const int64_t srow = PERM ? r : (P.row_order ? (int64_t)P.row_order[r] : r);
-        const int j0 = sub * E;
+        const int64_t srow = PERM ? r[u] : (P.row_order ? (int64_t)P.row_order[r[u]] : r[u]);
         const int64_t q = srow * (int64_t)k + j0;
-        int c[E];
-        double w[NV][E];
         if constexpr (E == 4) {
             const i32x4 c4 = *reinterpret_cast<const i32x4*>(P.col_idx + q);
-            c[0] = c4.x; c[1] = c4.y; c[2] = c4.z; c[3] = c4.w;
+            c[u][0] = c4.x; c[u][1] = c4.y; c[u][2] = c4.z; c[u][3] = c4.w;
         } else if constexpr (E == 2) {
             const i32x2 c2 = *reinterpret_cast<const i32x2*>(P.col_idx + q);
-            c[0] = c2.x; c[1] = c2.y;
+            c[u][0] = c2.x; c[u][1] = c2.y;
         } else {
-            c[0] = P.col_idx[q];
+            c[u][0] = P.col_idx[q];
         }
 #pragma unroll
         for (int v = 0; v < NV; ++v) {
@@ -435,33 +442,41 @@ __global__ __launch_bounds__(256) void k_spmv_vec(const SpmvDev P, int G_rt, int
 #pragma unroll
                 for (int h = 0; h < E / 2; ++h) {
                     const f64x2 t = *reinterpret_cast<const f64x2*>(vp + 2 * h);
-                    w[v][2 * h] = t.x;
-                    w[v][2 * h + 1] = t.y;
+                    w[u][v][2 * h] = t.x;
+                    w[u][v][2 * h + 1] = t.y;
                 }
             } else {
-                w[v][0] = vp[0];
+                w[u][v][0] = vp[0];
             }
         }
-        // E divides k (E = 1, or k a multiple of 2 or 4), so every lane's E entries
-        // lie in its row: the gathers are issued unconditionally, all in flight at once
-        double xv[E][FMAX];
+    }
+    // E divides k (E = 1, or k a multiple of 2 or 4), so every lane's E entries lie in
+    // its row: the gathers of all U units are issued unconditionally, all in flight at once
+    double xv[U][E][FMAX];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
 #pragma unroll
         for (int e = 0; e < E; ++e) {
             if constexpr (UNITX) {
-                xv[e][0] = P.x[c[e]];
+                xv[u][e][0] = P.x[c[u][e]];
             } else {
 #pragma unroll
                 for (int f = 0; f < FMAX; ++f)
-                    xv[e][f] = f < P.n_fields ? P.x[(int64_t)c[e] * P.x_ld_point + (int64_t)f * P.x_ld_field] : 0.0;
+                    xv[u][e][f] = f < P.n_fields ? P.x[(int64_t)c[u][e] * P.x_ld_point + (int64_t)f * P.x_ld_field] : 0.0;
             }
         }
+    double acc[U][NV][FMAX];
 #pragma unroll
-        for (int e = 0; e < E; ++e)
+    for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int v = 0; v < NV; ++v)
+        for (int v = 0; v < NV; ++v)
 #pragma unroll
-                for (int f = 0; f < FMAX; ++f) acc[v][f] += w[v][e] * xv[e][f];
-    }
+            for (int f = 0; f < FMAX; ++f) {
+                double t = 0.0;
+#pragma unroll
+                for (int e = 0; e < E; ++e) t += w[u][v][e] * xv[u][e][f];
+                acc[u][v][f] = t;
+            }
     // segmented tree sum over the row's G lanes: at distance o = 1, 2, 4, ... each lane
     // adds the partial of the lane o places on when that lane is in the same row, so
     // the row's first lane ends with the whole sum (lanes past the last row of the
@@ -471,22 +486,27 @@ __global__ __launch_bounds__(256) void k_spmv_vec(const SpmvDev P, int G_rt, int
         if (o >= G) break;  // uniform
         const bool in = sub + o < G;
 #pragma unroll
-        for (int v = 0; v < NV; ++v)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
-            for (int f = 0; f < FMAX; ++f) {
-                const double t = __shfl_down(acc[v][f], o, 64);
-                acc[v][f] += in ? t : 0.0;
-            }
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int f = 0; f < FMAX; ++f) {
+                    const double t = __shfl_down(acc[u][v][f], o, 64);
+                    acc[u][v][f] += in ? t : 0.0;
+                }
     }
-    if (live && sub == 0) {
-        const int64_t row = P.row_order ? (int64_t)P.row_order[r] : r;
+    if (sub != 0) return;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (!live[u]) continue;
+        const int64_t row = P.row_order ? (int64_t)P.row_order[r[u]] : r[u];
 #pragma unroll
         for (int v = 0; v < NV; ++v)
 #pragma unroll
             for (int f = 0; f < FMAX; ++f) {
                 if (f < P.n_fields) {
                     double* yp = P.y + (int64_t)v * P.y_ld_val + row * P.y_ld_point + (int64_t)f * P.y_ld_field;
-                    const double out = P.alpha * acc[v][f];
+                    const double out = P.alpha * acc[u][v][f];
                     *yp = (P.beta == 0.0) ? out : out + P.beta * *yp;
                 }
             }
@@ -507,21 +527,30 @@ int vec_entries(const SpmvDev& P, int nv)
     return (P.k + e - 1) / e <= 64 ? e : 0;
 }
 
+#ifndef RBFX_VEC_UNITS
+#define RBFX_VEC_UNITS 2
+#endif
+// units of rows per wavefront in the vector mapping: each lane holds the loads of U
+// rows in flight at once
+constexpr int kVecUnits = RBFX_VEC_UNITS;
+
 template <int NV, int FMAX, bool UNITX, bool PERM>
 static hipError_t spmv_vec_perm(const SpmvDev& P, int e, hipStream_t s)
 {
+    if (P.n_rows <= 0) return hipSuccess;  // the kernel reads row n_rows - 1
     const int G = (P.k + e - 1) / e;
-    const int64_t rows_per_block = 4 * (64 / G);
+    const int U = (e == 4 && G == 5) ? kVecUnits : 1;
+    const int64_t rows_per_block = 4 * U * (64 / G);
     const int64_t nbr = (P.n_rows + rows_per_block - 1) / rows_per_block;
     const dim3 grid((unsigned)pad8(nbr));
     if (e == 4 && G == 5)
-        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM, 5>), grid, dim3(256), 0, s, P, G, nbr);
+        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM, 5, kVecUnits>), grid, dim3(256), 0, s, P, G, nbr);
     else if (e == 4)
-        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM, 0>), grid, dim3(256), 0, s, P, G, nbr);
+        hipLaunchKernelGGL((k_spmv_vec<NV, 4, FMAX, UNITX, PERM, 0, 1>), grid, dim3(256), 0, s, P, G, nbr);
     else if (e == 2)
-        hipLaunchKernelGGL((k_spmv_vec<NV, 2, FMAX, UNITX, PERM, 0>), grid, dim3(256), 0, s, P, G, nbr);
+        hipLaunchKernelGGL((k_spmv_vec<NV, 2, FMAX, UNITX, PERM, 0, 1>), grid, dim3(256), 0, s, P, G, nbr);
     else
-        hipLaunchKernelGGL((k_spmv_vec<NV, 1, FMAX, UNITX, PERM, 0>), grid, dim3(256), 0, s, P, G, nbr);
+        hipLaunchKernelGGL((k_spmv_vec<NV, 1, FMAX, UNITX, PERM, 0, 1>), grid, dim3(256), 0, s, P, G, nbr);
     return hipGetLastError();
 }
 
