@@ -421,6 +421,51 @@ def test_reorder_points(RBF, gpu):
     assert np.array_equal(qt.cpu().numpy(), cloud[perm_t])
 
 
+@pytest.mark.parametrize("fields", [1, 3])
+def test_vector_apply_row_tails(RBF, gpu, fields):
+    """The vector mapping with k = 20 runs two units of 12 rows per wavefront and 96 rows
+    per workgroup; lanes of rows past the end read the last row and write nothing.
+    Evaluation sets of 1 to 97 rows give the CSR product, with alpha and beta scaling
+    and accumulating into a strided Y whose rows beyond the set stay untouched. A
+    70,001-row set is applied in spatial row order both through the apply plan (weights
+    permuted once) and without it (entry addresses taken through row_order)."""
+    import torch
+
+    rng = np.random.default_rng(21)
+    data = rng.random((400, 2))
+    for n_eval in (1, 11, 12, 13, 24, 25, 95, 97):
+        ev = rng.random((n_eval, 2)) * 0.9 + 0.05
+        op = RBF.laplacian(data, basis=RBF.PHS(3, poly_deg=2), k=20, eval_points=ev, device=gpu)
+        op.SPMV_MAPPING = 3
+        W = op.weights()
+        Fh = rng.standard_normal((len(data), fields))
+        Y0 = rng.standard_normal((n_eval, fields))
+        big = torch.full((1, n_eval + 5, fields), 7.0, dtype=torch.float64, device=gpu)
+        big[0, :n_eval] = torch.as_tensor(Y0, device=gpu)
+        op._spmv(torch.as_tensor(Fh, device=gpu), out=big[:, :n_eval], alpha=2.0, beta=0.5)
+        got = big.cpu().numpy()
+        ref = 2.0 * (W @ Fh) + 0.5 * Y0
+        bound = 1e-13 * (2.0 * (abs(W) @ np.abs(Fh)) + 0.5 * np.abs(Y0))
+        assert np.all(np.abs(got[0, :n_eval] - ref) <= bound), n_eval
+        assert np.all(got[0, n_eval:] == 7.0), n_eval
+
+    n = 70_001  # above the spatial-order threshold, not a multiple of 96
+    pts = rng.random((n, 2))
+    op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, device=gpu)
+    op.SPMV_MAPPING = 3
+    W = op.weights()
+    Fh = rng.standard_normal((n, fields))
+    F = torch.as_tensor(Fh, device=gpu)
+    ref = W @ Fh
+    bound = 1e-13 * (abs(W) @ np.abs(Fh))
+    y_direct = op._spmv(F, asynchronous=True)[0].cpu().numpy()
+    assert op._plan is None
+    y_plan = op._spmv(F)[0].cpu().numpy()
+    assert op._plan is not None
+    assert np.all(np.abs(y_direct - ref) <= bound)
+    assert np.all(np.abs(y_plan - ref) <= bound)
+
+
 @pytest.mark.parametrize("fields", [1, 3, 6, 16, 40])
 def test_spmv_mappings_agree(RBF, gpu, fields):
     """The apply mappings (lanes per row reading HBM; rows staged through LDS; vector
