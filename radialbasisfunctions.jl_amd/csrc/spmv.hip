@@ -389,9 +389,10 @@ static int staged_g(int64_t L) { return L <= 20 ? 4 : (L <= 40 ? 8 : 16); }
 // Vector mapping, for rows of one length k (no row_ptr) and few fields: G = ceil(k / E)
 // lanes per row, each owning E consecutive entries, which it reads with one 4E-byte
 // index load and E/2 16-byte value loads per value array (E = 4 or 2; E = 1: scalar
-// loads). A wavefront holds floor(64 / G) rows, so its loads cover one contiguous
-// run of rows and read every byte of it once, without an LDS round trip. The G
-// partial sums of a row are added by the row's first lane.
+// loads). A wavefront holds U units of floor(64 / G) consecutive rows (U = 2 for
+// k = 20, else 1), so its loads cover one contiguous run of rows and read every byte
+// of it once, without an LDS round trip; each lane has the loads of its U rows in
+// flight together. The G partial sums of a row are added by the row's first lane.
 typedef int i32x2 __attribute__((ext_vector_type(2)));
 
 template <int NV, int E, int FMAX, bool UNITX, bool PERM, int GS, int U>
