@@ -283,11 +283,22 @@ static hipError_t knn_dim(KnnWorkspace& ws, const double* data, int64_t n_data, 
 
     const int QB = 128;
     const dim3 gq((unsigned)((n_query + QB - 1) / QB));
+    // the insertion network costs KCAP - 1 steps per accepted candidate, so KCAP
+    // follows k closely in the common range
     if (k <= 16)
         hipLaunchKernelGGL((k_query<DIM, 16>), gq, dim3(QB), 0, st, ws.sx, ws.sidx, ws.start, query, ws.qorder,
                            n_query, G, k, q_ptr, out);
+    else if (k <= 20)
+        hipLaunchKernelGGL((k_query<DIM, 20>), gq, dim3(QB), 0, st, ws.sx, ws.sidx, ws.start, query, ws.qorder,
+                           n_query, G, k, q_ptr, out);
+    else if (k <= 24)
+        hipLaunchKernelGGL((k_query<DIM, 24>), gq, dim3(QB), 0, st, ws.sx, ws.sidx, ws.start, query, ws.qorder,
+                           n_query, G, k, q_ptr, out);
     else if (k <= 32)
         hipLaunchKernelGGL((k_query<DIM, 32>), gq, dim3(QB), 0, st, ws.sx, ws.sidx, ws.start, query, ws.qorder,
+                           n_query, G, k, q_ptr, out);
+    else if (k <= 40)
+        hipLaunchKernelGGL((k_query<DIM, 40>), gq, dim3(QB), 0, st, ws.sx, ws.sidx, ws.start, query, ws.qorder,
                            n_query, G, k, q_ptr, out);
     else if (k <= 48)
         hipLaunchKernelGGL((k_query<DIM, 48>), gq, dim3(QB), 0, st, ws.sx, ws.sidx, ws.start, query, ws.qorder,

@@ -60,7 +60,8 @@ def _gpu_weights(op):
 
 
 # ----------------------------------------------------------------- k-NN
-@pytest.mark.parametrize("dim,k,nq", [(2, 20, 3000), (3, 40, 2000), (2, 60, 1000), (1, 8, 500), (3, 30, 2500)])
+@pytest.mark.parametrize("dim,k,nq", [(2, 20, 3000), (3, 40, 2000), (2, 60, 1000), (1, 8, 500), (3, 30, 2500),
+                                      (2, 17, 1500), (3, 22, 1500), (1, 24, 500), (2, 36, 1000)])
 def test_knn_matches_bruteforce(RBF, O, gpu, dim, k, nq):
     rng = np.random.default_rng(100 + dim * k)
     pts = rng.random((6000, dim))
