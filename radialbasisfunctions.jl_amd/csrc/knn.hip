@@ -177,17 +177,17 @@ __global__ __launch_bounds__(128) void k_query(const double* __restrict__ sx, co
                         const double d2 = sqdist<DIM>(sx + (int64_t)s * DIM, q);
                         const int id = sidx[s];
                         if (!lt(d2, id, bd[KCAP - 1], bi[KCAP - 1])) continue;
-                        // insertion network on the sorted list (old values read right to left)
+                        // insertion position p = number of entries preceding the candidate
+                        // (the list is sorted, so they are slots [0, p)); then shift right
+                        int p = 0;
+#pragma unroll
+                        for (int u = 0; u < KCAP - 1; ++u) p += lt(bd[u], bi[u], d2, id) ? 1 : 0;
 #pragma unroll
                         for (int u = KCAP - 1; u >= 1; --u) {
-                            const bool before_prev = lt(d2, id, bd[u - 1], bi[u - 1]);
-                            const bool before_cur = lt(d2, id, bd[u], bi[u]);
-                            const double nd = before_prev ? bd[u - 1] : (before_cur ? d2 : bd[u]);
-                            const int ni = before_prev ? bi[u - 1] : (before_cur ? id : bi[u]);
-                            bd[u] = nd;
-                            bi[u] = ni;
+                            bd[u] = u > p ? bd[u - 1] : (u == p ? d2 : bd[u]);
+                            bi[u] = u > p ? bi[u - 1] : (u == p ? id : bi[u]);
                         }
-                        if (lt(d2, id, bd[0], bi[0])) { bd[0] = d2; bi[0] = id; }
+                        if (p == 0) { bd[0] = d2; bi[0] = id; }
                         ++filled;
                     }
                 }
