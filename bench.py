@@ -148,7 +148,8 @@ def time_apply(R, op, X, stream, reps=20):
 
 
 # ------------------------------------------------------------------ other configurations
-KERNEL_NAMES = {-1: "-", 0: "regs x2", 1: "regs", 2: "lds", 3: "regs x2 (fp64)", 4: "regs (fp64)"}
+KERNEL_NAMES = {-1: "-", 0: "regs x2", 1: "regs", 2: "lds", 3: "regs x2 (fp64)", 4: "regs (fp64)",
+                5: "null-space cholesky", 6: "null-space cholesky x2"}
 
 
 def _build_ms(R, op, reps=2):

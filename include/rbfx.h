@@ -313,8 +313,33 @@ int rbfx_last_build_timing(double ms[4]);
 int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3]);
 
 /* Pin the kernel of a size bucket for later builds in this process (kernel -1:
- * automatic choice again). For tests and tuning; no reference counterpart.      */
+ * automatic choice again). For tests and tuning; no reference counterpart.
+ * Kernel ids: 0 rook LDL^T in registers, two stencils per wavefront (n <= 32);
+ * 1 the same, one stencil per wavefront (n <= 64); 2 rook LDL^T with the system in
+ * LDS; 3, 4 kernels 0, 1 in FP64 within
+ * an FP32 + refinement build; 5 null-space Cholesky with one FP64 refinement step
+ * (interior stencils of bases definite on the polynomial null space: PHS1 with degree
+ * >= 0, PHS3 >= 1, PHS5 >= 2, PHS7 >= 3, IMQ, Gaussian; the stencils whose pivots or
+ * refinement fail are re-solved by kernel 0 or 2); 6 the same with two rows per lane
+ * (k <= 20, at most 6 monomials). A pinned kernel that cannot serve a build is
+ * ignored by that build.                                                        */
 int rbfx_tune_bucket(int32_t bucket, int32_t kernel);
+
+/* Counters of the last rbfx_build_weights on this thread (no reference counterpart):
+ * [0] stencils the null-space kernels handed to the rook kernels (breakdown, refinement
+ * not settling, Hermite or out-of-shape stencils), [1] autotuner sample launches, the
+ * rest 0.                                                                        */
+int rbfx_last_build_stats(int64_t stats[8]);
+
+/* Save / load the autotuner's choices (no reference counterpart; the checkpoint of
+ * the tuning map): one text line per configuration (dimension, operator columns,
+ * precision, size bucket, largest k, monomials, basis kind, boundary data, kernel).
+ * A loaded choice is used instead of timing samples, on every device, for the
+ * configurations it names.                                                       */
+int rbfx_tune_save(const char* path);
+int rbfx_tune_load(const char* path);
+/* Forget every choice made or loaded so far: later builds time their samples again. */
+int rbfx_tune_reset(void);
 
 /* ---- surrogate training and inputs (no reference counterpart) ------------ */
 

@@ -679,11 +679,15 @@ static int build_stencil(const rbfx_build_args* a, int64_t e, int scaled, int us
     const int r = a->op.n_cols;
     const double* xe = a->eval + e * dim;
 
-    /* classify_stencil, types.jl:208-223 */
+    /* classify_stencil, types.jl:208-223. A Dirichlet eval node is classified before
+     * the neighbour scan: count_nonzeros (execution.jl:39-69) gives it a one-entry row
+     * whatever its neighbours are, and the reference's neighbour-first order would, for
+     * a stencil holding no boundary point, fill k entries into that row (a write past
+     * it under @inbounds). Every other stencil is classified as the reference does. */
     int anyb = 0;
     if (a->is_boundary)
         for (int i = 0; i < k; ++i) anyb |= a->is_boundary[nb[i]] ? 1 : 0;
-    if (anyb && a->is_boundary[e]) {
+    if (a->is_boundary && e < a->n_data && a->is_boundary[e]) {
         pinfo pe = point_info(a, e);
         if (pe.type == PT_DIRICHLET) return RBFX_ST_DIRICHLET;
     }

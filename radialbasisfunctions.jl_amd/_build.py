@@ -6,6 +6,7 @@ linked into radialbasisfunctions.jl_amd/librbfx.so.
 import concurrent.futures as cf
 import os
 import pathlib
+import re
 import subprocess
 import sys
 import time
@@ -27,6 +28,7 @@ FLAGS = [
     "-Wno-unused-result",
     "-I", str(ROOT / "include"),
     "-I", str(CSRC),
+    "-Rpass-analysis=kernel-resource-usage",
 ]
 
 
@@ -37,9 +39,38 @@ def _compile(src, verbose):
     p = subprocess.run(cmd, capture_output=True, text=True)
     if p.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{p.stderr[-6000:]}")
+    (OBJ / (src.stem + ".remarks")).write_text(p.stderr)
     if verbose:
         print(f"  compiled {src.name} in {time.time() - t0:.1f}s", flush=True)
     return obj
+
+
+def resource_usage():
+    """Per-kernel register use from the compiler's remarks of the last build:
+    {kernel: {"VGPRs", "ScratchSize", "VGPRs Spill", "Occupancy"}}."""
+    out = {}
+    for f in sorted(OBJ.glob("*.remarks")):
+        cur = None
+        for line in f.read_text().splitlines():
+            m = re.search(r"remark: +([A-Za-z ]+?)(?: \[bytes/lane\]| \[waves/SIMD\])?: (\S+)", line)
+            if not m:
+                continue
+            key, val = m.group(1).strip(), m.group(2)
+            if key == "Function Name":
+                cur = out.setdefault(val, {})
+            elif cur is not None and key in ("VGPRs", "ScratchSize", "VGPRs Spill", "Occupancy"):
+                cur[key] = int(val)
+    return out
+
+
+def write_resource_table(path):
+    rows = resource_usage()
+    lines = ["kernel  VGPRs  scratch_bytes_per_lane  vgpr_spills  waves_per_SIMD"]
+    for name, r in sorted(rows.items()):
+        if name.startswith("_ZN4rbfx"):
+            lines.append(f"{name}  {r.get('VGPRs')}  {r.get('ScratchSize')}  {r.get('VGPRs Spill')}  {r.get('Occupancy')}")
+    pathlib.Path(path).write_text("\n".join(lines) + "\n")
+    return rows
 
 
 def build_library(jobs=None, verbose=True):
@@ -54,8 +85,9 @@ def build_library(jobs=None, verbose=True):
     if p.returncode != 0:
         raise RuntimeError(f"link failed:\n{p.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    write_resource_table(OBJ / "resource_usage.txt")
     if verbose:
-        print(f"  linked {LIB}", flush=True)
+        print(f"  linked {LIB}; per-kernel registers in {OBJ / 'resource_usage.txt'}", flush=True)
     return LIB
 
 

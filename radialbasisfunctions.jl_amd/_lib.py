@@ -61,6 +61,13 @@ def lib():
                                                 C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p]
                 L.rbfx_spatial_order.restype = C.c_int
                 L.rbfx_spatial_order.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
+                L.rbfx_last_build_stats.restype = C.c_int
+                L.rbfx_last_build_stats.argtypes = [C.POINTER(C.c_int64)]
+                L.rbfx_tune_save.restype = C.c_int
+                L.rbfx_tune_save.argtypes = [C.c_char_p]
+                L.rbfx_tune_load.restype = C.c_int
+                L.rbfx_tune_load.argtypes = [C.c_char_p]
+                L.rbfx_tune_reset.restype = C.c_int
                 L.rbfx_last_build_timing.restype = C.c_int
                 L.rbfx_last_build_timing.argtypes = [C.POINTER(C.c_double)]
                 L.rbfx_mlp_param_count.restype = C.c_int64
@@ -87,7 +94,8 @@ def exported_symbols():
             "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
             "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async",
-            "rbfx_permute_rows", "rbfx_last_build_buckets", "rbfx_tune_bucket"]
+            "rbfx_permute_rows", "rbfx_last_build_buckets", "rbfx_tune_bucket", "rbfx_last_build_stats",
+            "rbfx_tune_save", "rbfx_tune_load", "rbfx_tune_reset"]
 
 
 def _check(rc):
@@ -161,8 +169,34 @@ def last_build_buckets():
 
 
 def tune_bucket(bucket, kernel):
-    """Pin the kernel of a size bucket (-1: automatic)."""
+    """Pin the kernel of a size bucket (-1: automatic). Kernel ids as rbfx_tune_bucket
+    documents them: 0 rook registers x2, 1 rook registers (pinned only), 2 rook LDS,
+    3/4 those in FP64 within FP32 mode, 5 null-space Cholesky, 6 the same, two rows
+    per lane."""
     _check(lib().rbfx_tune_bucket(int(bucket), int(kernel)))
+
+
+def last_build_stats():
+    """[stencils the null-space kernels handed to the rook kernels, autotuner sample
+    launches, 0, ...] of the last build on this thread."""
+    st = (C.c_int64 * 8)()
+    _check(lib().rbfx_last_build_stats(st))
+    return list(st)
+
+
+def tune_save(path):
+    """Write the autotuner's choices (every device) to a text file."""
+    _check(lib().rbfx_tune_save(str(path).encode()))
+
+
+def tune_load(path):
+    """Use the choices of a file written by tune_save instead of timing samples."""
+    _check(lib().rbfx_tune_load(str(path).encode()))
+
+
+def tune_reset():
+    """Forget every autotuner choice made or loaded in this process."""
+    _check(lib().rbfx_tune_reset())
 
 
 def spmv(args, dev, asynchronous=False):

@@ -11,11 +11,13 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <mutex>
 #include <string>
 #include <vector>
 
 #include "build.h"
+#include "build_nsc.h"
 #include "knn.h"
 #include "learned.h"
 #include "mlp.h"
@@ -46,6 +48,11 @@ int set_err(int code, const char* fmt, ...)
         if (e_ != hipSuccess) return set_err(RBFX_EDEVICE, "%s failed: %s", #x, hipGetErrorString(e_)); \
     } while (0)
 
+// Autotuner configuration key: dimension, operator columns, precision mode, size
+// bucket, largest stencil (k points, m monomials), basis kind and whether the build has
+// boundary data.
+typedef std::tuple<int, int, int, int, int, int, int, int> TuneKey;
+
 // per-device library scratch (grow-only)
 struct DeviceState {
     rbfx::KnnWorkspace knn;
@@ -64,13 +71,17 @@ struct DeviceState {
     size_t cap_bucket = 0;
     int32_t* bucket_count = nullptr;  // [3] list lengths, [3..5] tuning sample sizes
     hipEvent_t tev[2] = {nullptr, nullptr};
-    std::map<int, int> tuned;  // (dim, columns, precision, bucket) -> kernel
+    std::map<TuneKey, int> tuned;  // configuration -> kernel the autotuner picked
+    int32_t* nsc_list = nullptr;   // stencils the null-space kernels hand back
+    size_t cap_nsc = 0;
+    int32_t* nsc_count = nullptr;
     double* packed = nullptr;  // per-position stencil offsets of a register-kernel launch
     size_t cap_packed = 0;
 };
 
 std::mutex g_mu;
 DeviceState g_dev[64];
+std::map<TuneKey, int> g_tune_saved;  // choices loaded with rbfx_tune_load (any device)
 
 DeviceState& dev_state()
 {
@@ -220,56 +231,137 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
 
 
 // ---------------------------------------------------------------- size buckets
-// Kernels a bucket can run on: the register kernel with two stencils per wavefront
-// (n <= 32), the register kernel with one (n <= 64), the LDS kernel (n <= RBFX_MAX_N,
-// FP64), and, in FP32 + refinement mode, the two register kernels in FP64.
-enum { K_REG32 = 0, K_REG64 = 1, K_WIDE = 2, K_REG32_F64 = 3, K_REG64_F64 = 4, kNumKernels = 5 };
+// Kernels a bucket can run on:
+//   K_REG32   rook LDL^T in registers, two stencils per wavefront (n <= 32)
+//   K_REG64   the same, one stencil per wavefront (n <= 64; one wave per SIMD, the
+//             register overflow held in AGPRs, no scratch memory)
+//   K_WIDE    rook LDL^T with the system in LDS (n <= RBFX_MAX_N)
+//   K_REG32_F64, K_REG64_F64  the register kernels in FP64 within an FP32 + refinement
+//             build
+//   K_NSC, K_NSC2  null-space Cholesky (no pivot search) with one FP64 refinement
+//             step, for interior stencils of bases definite on the polynomial null
+//             space; the stencils it hands back are re-solved by K_REG32 / K_WIDE.
+//             K_NSC2 maps two rows per lane (four stencils per wavefront, k <= 20).
+enum {
+    K_REG32 = 0,
+    K_REG64 = 1,
+    K_WIDE = 2,
+    K_REG32_F64 = 3,
+    K_REG64_F64 = 4,
+    K_NSC = 5,
+    K_NSC2 = 6,
+    kNumKernels = 7
+};
 int g_force_kernel[3] = {-1, -1, -1};  // rbfx_tune_bucket overrides
 thread_local int64_t g_bucket_n[3];
 thread_local int32_t g_bucket_kernel[3];
+thread_local int64_t g_stats[8];       // rbfx_last_build_stats
 constexpr int64_t kTuneMin = 4096;     // stencils in a bucket before its mapping is timed
 constexpr int32_t kTuneSample = 16384;
+
+// What decides which kernels can serve a launch: the largest stencil of the bucket
+// (kmax points, m monomials), the operator columns, the basis and whether the build has
+// boundary data (Hermite stencils are never solved by the null-space kernels; they hand
+// them back).
+struct Shape {
+    int dim, kmax, m, n_cols, basis_kind, poly_deg;
+    bool hermite;
+};
+
+static bool is_nsc(int kid) { return kid == K_NSC || kid == K_NSC2; }
 
 // In FP32 + refinement mode the FP64 kernels are candidates too (their stencils get
 // status FP64): with packed FP32 arithmetic unused, FP32 issues at the FP64 rate on
 // gfx950 and the FP32 kernels hold extra registers for the refinement, so an FP64
 // solve can be the faster one for a size class; the tuner measures which.
-static bool kernel_serves(int kid, int bucket, bool fp32, int n_cols)
+static bool kernel_serves(int kid, int bucket, bool fp32, const Shape& sh, bool pinned)
 {
-    if (kid == K_REG32) return bucket == 0;
-    if (kid == K_REG64) return bucket <= 1;
-    if (kid == K_WIDE) return bucket == 2 || (bucket == 1 && rbfx::wide_fits(64, n_cols));
-    if (kid == K_REG32_F64) return fp32 && bucket == 0;
-    if (kid == K_REG64_F64) return fp32 && bucket <= 1;
-    return false;
+    switch (kid) {
+    case K_REG32: return bucket == 0;
+    // (in FP32 mode the 3-D kernel with 8 columns still spills at one wave: not served)
+    case K_REG64: return bucket <= 1 && !(fp32 && sh.dim == 3 && sh.n_cols > 4);
+    case K_WIDE: return bucket == 2 || (bucket == 1 && rbfx::wide_fits(64, sh.n_cols));
+    case K_REG32_F64: return fp32 && bucket == 0;
+    case K_REG64_F64: return fp32 && bucket <= 1;
+    case K_NSC:
+    case K_NSC2: {
+        if (rbfx::nsc_sign(sh.basis_kind, sh.poly_deg) == 0) return false;
+        const int cap = bucket == 0 ? 32 : (bucket == 1 ? 64 : RBFX_MAX_N);
+        const int kb = std::min(sh.kmax, cap);
+        return rbfx::nsc_fits(sh.dim, kb, sh.m, sh.n_cols, kid == K_NSC2 ? 1 : 0);
+    }
+    default: return false;
+    }
 }
 
-static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, int dim, bool fp32, int rows_cap, int64_t n_work,
-                                   hipStream_t st)
+// The rook kernel that re-solves what a null-space launch handed back (FP64).
+static int rook_for(int bucket, int n_cols)
+{
+    if (bucket == 0) return K_REG32;
+    return rbfx::wide_fits(bucket == 1 ? 64 : RBFX_MAX_N, n_cols) ? K_WIDE : K_REG64;
+}
+
+static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape& sh, DeviceState& ds, int bucket,
+                                   bool fp32, int rows_cap, int64_t n_work, bool async, hipStream_t st)
 {
     rbfx::BuildDev Q = P;
-    if (fp32 && (kid == K_WIDE || kid == K_REG32_F64 || kid == K_REG64_F64)) Q.ok_status = RBFX_ST_FP64;
-    if (kid == K_WIDE) return rbfx::launch_build_wide(Q, dim, rows_cap, n_work, st);
+    if (fp32 && (kid == K_WIDE || kid == K_REG32_F64 || kid == K_REG64_F64 || is_nsc(kid))) Q.ok_status = RBFX_ST_FP64;
+    if (is_nsc(kid)) {
+        // the null-space launch, then the rook kernel on the stencils it handed back
+        const int cap = bucket == 0 ? 32 : (bucket == 1 ? 64 : RBFX_MAX_N);
+        Q.fail_list = ds.nsc_list;
+        Q.fail_count = ds.nsc_count;
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.nsc_count);
+        hipError_t e = rbfx::launch_build_nsc(Q, sh.dim, std::min(sh.kmax, cap), sh.m, kid == K_NSC2 ? 1 : 0, n_work, st);
+        if (e != hipSuccess) return e;
+        int32_t nf = (int32_t)n_work;
+        if (!async) {
+            e = read_count(ds.nsc_count, &nf, st);
+            if (e != hipSuccess) return e;
+            g_stats[0] += nf;
+        }
+        if (nf <= 0) return hipSuccess;
+        rbfx::BuildDev F = Q;
+        F.fail_list = P.fail_list;  // (FP32 mode: the caller's list, unused by FP64 kernels)
+        F.fail_count = P.fail_count;
+        F.list = ds.nsc_list;
+        F.list_len = ds.nsc_count;
+        F.packed = nullptr;         // packed offsets follow the launch's own work order
+        return launch_kernel_id(rook_for(bucket, Q.n_cols), F, sh, ds, bucket, false, rows_cap, nf, async, st);
+    }
+    if (kid == K_WIDE) return rbfx::launch_build_wide(Q, sh.dim, rows_cap, n_work, st);
     const bool f32 = fp32 && (kid == K_REG32 || kid == K_REG64);
-    return rbfx::launch_build(Q, dim, (kid == K_REG32 || kid == K_REG32_F64) ? 32 : 64, f32, n_work, st);
+    return rbfx::launch_build(Q, sh.dim, (kid == K_REG32 || kid == K_REG32_F64) ? 32 : 64, f32, n_work, st);
 }
 
 // Kernel for a size bucket: forced (rbfx_tune_bucket), cached for this configuration,
 // or timed on a sample of the bucket's stencils (the autotuner: each kernel that serves
 // the bucket runs the sample twice, the second run timed; the fastest is kept), or the
-// narrowest kernel that fits. Q describes the sample: a work list whose length is
-// sample_len (uneven builds), or the first `sample` stencils in processing order.
-static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, DeviceState& ds, int bucket, bool fp32,
-                       int rows_cap, int64_t count, int32_t* sample_len, bool async, hipStream_t st, int* kid)
+// default (a null-space kernel where one serves, else the narrowest rook kernel that
+// fits). Q describes the sample: a work list whose length is sample_len (uneven
+// builds), or the first `sample` stencils in processing order.
+static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const Shape& sh, DeviceState& ds, int bucket,
+                       bool fp32, int rows_cap, int64_t count, int32_t* sample_len, bool async, hipStream_t st,
+                       int* kid)
 {
-    const int dflt = bucket == 0 ? K_REG32 : (bucket == 1 ? K_REG64 : K_WIDE);
+    int dflt = bucket == 0 ? K_REG32 : K_WIDE;
+    if (bucket == 1 && !rbfx::wide_fits(64, sh.n_cols)) dflt = K_REG64;
+    if (kernel_serves(K_NSC, bucket, fp32, sh, false)) dflt = K_NSC;
     *kid = dflt;
-    if (g_force_kernel[bucket] >= 0 && kernel_serves(g_force_kernel[bucket], bucket, fp32, Q.n_cols)) {
-        *kid = g_force_kernel[bucket];
+    const int forced = g_force_kernel[bucket];
+    if (forced >= 0 && kernel_serves(forced, bucket, fp32, sh, true)) {
+        *kid = forced;
         return RBFX_OK;
     }
-    const int key = ((a->dim * 16 + Q.n_cols) * 4 + a->precision) * 4 + bucket;
+    const TuneKey key{a->dim, sh.n_cols, a->precision, bucket, sh.kmax, sh.m, sh.basis_kind, sh.hermite ? 1 : 0};
     auto it = ds.tuned.find(key);
+    if (it == ds.tuned.end()) {
+        auto jt = g_tune_saved.find(key);  // a map loaded with rbfx_tune_load
+        if (jt != g_tune_saved.end() && kernel_serves(jt->second, bucket, fp32, sh, false)) {
+            ds.tuned[key] = jt->second;
+            it = ds.tuned.find(key);
+        }
+    }
     if (it != ds.tuned.end()) {
         *kid = it->second;
         return RBFX_OK;
@@ -281,7 +373,7 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, Device
     int cands[kNumKernels], nc = 0;
     for (int pass = 0; pass < 2; ++pass)
         for (int c = 0; c < kNumKernels; ++c)
-            if (kernel_serves(c, bucket, fp32, Q.n_cols) && f32_factor(c) == (pass == 1)) cands[nc++] = c;
+            if (kernel_serves(c, bucket, fp32, sh, false) && f32_factor(c) == (pass == 1)) cands[nc++] = c;
     if (nc < 2) {
         ds.tuned[key] = dflt;
         return RBFX_OK;
@@ -297,12 +389,13 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, Device
     float best = 0.f, f64_per_stencil = -1.f;
     for (int c = 0; c < nc; ++c) {
         const bool f32c = f32_factor(cands[c]) && Q.fail_count;
-        HIPCHK(launch_kernel_id(cands[c], S, a->dim, fp32, rows_cap, sample, st));  // warm-up
+        HIPCHK(launch_kernel_id(cands[c], S, sh, ds, bucket, fp32, rows_cap, sample, false, st));  // warm-up
         if (f32c) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, Q.fail_count);
         HIPCHK(hipEventRecord(ds.tev[0], st));
-        HIPCHK(launch_kernel_id(cands[c], S, a->dim, fp32, rows_cap, sample, st));
+        HIPCHK(launch_kernel_id(cands[c], S, sh, ds, bucket, fp32, rows_cap, sample, false, st));
         HIPCHK(hipEventRecord(ds.tev[1], st));
         HIPCHK(hipEventSynchronize(ds.tev[1]));
+        g_stats[1] += 2;
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, ds.tev[0], ds.tev[1]));
         if (f32c) {
@@ -319,6 +412,7 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, Device
             *kid = cands[c];
         }
     }
+    g_stats[0] = 0;  // the sample runs' hand-backs are not the build's
     ds.tuned[key] = *kid;
     return RBFX_OK;
 }
@@ -348,17 +442,35 @@ static const double* pack_offsets(const rbfx_build_args* a, const rbfx::BuildDev
     return ds.packed;
 }
 
+// The hand-back list of the null-space kernels (n_eval entries), grown by blocking builds.
+static int reserve_nsc_list(DeviceState& ds, int64_t n, bool async)
+{
+    const size_t need = sizeof(int32_t) * (size_t)std::max<int64_t>(n, 1);
+    if (async) {
+        if (ds.cap_nsc < need || !ds.nsc_count)
+            return set_err(RBFX_EINVAL, "asynchronous build: run one blocking build with these sizes first");
+        return RBFX_OK;
+    }
+    HIPCHK(grow((void**)&ds.nsc_list, ds.cap_nsc, need));
+    if (!ds.nsc_count) HIPCHK(hipMalloc((void**)&ds.nsc_count, sizeof(int32_t)));
+    return RBFX_OK;
+}
+
 // Main solve of all stencils (plain FP64, or FP32 + refinement). Stencils of one size
 // run in one launch. Uneven stencils (ragged k, per-stencil degree) are classified into
 // size buckets in processing order, and each bucket runs on the kernel chosen for it.
 // (A tuning sample's queued FP32 failures are discarded with the old count.)
-static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceState& ds, int nmax, bool fp32,
-                     bool async, hipStream_t st)
+static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Shape& sh, DeviceState& ds, int nmax,
+                     bool fp32, bool async, hipStream_t st)
 {
     const int dim = a->dim;
     for (int b = 0; b < 3; ++b) {
         g_bucket_n[b] = 0;
         g_bucket_kernel[b] = -1;
+    }
+    {
+        const int rc = reserve_nsc_list(ds, a->n_eval, async);
+        if (rc != RBFX_OK) return rc;
     }
     const bool uneven = (a->adj_ptr || a->poly_deg_map) && nmax > 32;
     if (!uneven) {
@@ -368,12 +480,12 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceSt
         int prc = RBFX_OK;
         Q.packed = pack_offsets(a, P, ds, nmax, a->n_eval, async, st, &prc);
         if (prc != RBFX_OK) return prc;
-        const int rc = pick_kernel(a, Q, ds, b, fp32, nmax, a->n_eval, nullptr, async, st, &kid);
+        const int rc = pick_kernel(a, Q, sh, ds, b, fp32, nmax, a->n_eval, nullptr, async, st, &kid);
         if (rc != RBFX_OK) return rc;
         g_bucket_n[b] = a->n_eval;
         g_bucket_kernel[b] = kid;
         if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
-        HIPCHK(launch_kernel_id(kid, Q, dim, fp32, nmax, a->n_eval, st));
+        HIPCHK(launch_kernel_id(kid, Q, sh, ds, b, fp32, nmax, a->n_eval, async, st));
         return RBFX_OK;
     }
     const size_t need = sizeof(int32_t) * 3 * (size_t)a->n_eval;
@@ -398,7 +510,8 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceSt
         rbfx::BuildDev Q = P;
         Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
         Q.list_len = ds.bucket_count + 3 + b;
-        const int rc = pick_kernel(a, Q, ds, b, fp32, rows_cap[b], cnt[b], ds.bucket_count + 3 + b, async, st, &kid[b]);
+        const int rc =
+            pick_kernel(a, Q, sh, ds, b, fp32, rows_cap[b], cnt[b], ds.bucket_count + 3 + b, async, st, &kid[b]);
         if (rc != RBFX_OK) return rc;
     }
     if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
@@ -409,7 +522,7 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, DeviceSt
         rbfx::BuildDev Q = P;
         Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
         Q.list_len = ds.bucket_count + b;
-        HIPCHK(launch_kernel_id(kid[b], Q, dim, fp32, rows_cap[b], cnt[b], st));
+        HIPCHK(launch_kernel_id(kid[b], Q, sh, ds, b, fp32, rows_cap[b], cnt[b], async, st));
     }
     return RBFX_OK;
 }
@@ -419,6 +532,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
     if (!a) return set_err(RBFX_EINVAL, "rbfx_build_weights: null args");
     for (int i = 0; i < 16; ++i) g_counts[i] = 0;
     for (int i = 0; i < 4; ++i) g_stage_ms[i] = 0.0;
+    for (int i = 0; i < 8; ++i) g_stats[i] = 0;
     if (a->dim < 1 || a->dim > 3) return set_err(RBFX_EINVAL, "dim must be 1, 2 or 3 (got %d)", a->dim);
     if (!a->data || !a->eval || !a->adj_idx || !a->out_val)
         return set_err(RBFX_EINVAL, "rbfx_build_weights: data, eval, adj_idx and out_val are required");
@@ -556,6 +670,14 @@ static int build_impl(const rbfx_build_args* a, bool async)
     P.ok_status = RBFX_ST_OK;
 
     const int nmax = a->k + M;
+    Shape sh;
+    sh.dim = a->dim;
+    sh.kmax = a->k;
+    sh.m = M;
+    sh.n_cols = op.n_cols;
+    sh.basis_kind = kind;
+    sh.poly_deg = deg;
+    sh.hermite = a->is_boundary != nullptr;
     // stage boundaries: ev0 | main kernel | ev1 | FP32 list solve | ev2 | FP64 fallback | ev3
     // (blocking builds only; an asynchronous build records nothing but kernels)
     auto mark = [&](int i) -> hipError_t {
@@ -568,7 +690,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
     };
     HIPCHK(mark(0));
     if (a->precision == RBFX_PREC_FP64) {
-        const int rc = solve_all(a, P, ds, nmax, false, async, st);
+        const int rc = solve_all(a, P, sh, ds, nmax, false, async, st);
         if (rc != RBFX_OK) return rc;
         HIPCHK(mark(1));
         HIPCHK(mark(2));
@@ -633,8 +755,12 @@ static int build_impl(const rbfx_build_args* a, bool async)
             int prc = RBFX_OK;
             Q.packed = pack_offsets(a, P, ds, nmax, async ? a->n_eval : n_solve, async, st, &prc);
             if (prc != RBFX_OK) return prc;
-            const int rc = pick_kernel(a, Q, ds, b, true, nmax, n_solve, ds.bucket_count ? ds.bucket_count + 3 : nullptr,
-                                       async, st, &kid);
+            {
+                const int rr = reserve_nsc_list(ds, a->n_eval, async);
+                if (rr != RBFX_OK) return rr;
+            }
+            const int rc = pick_kernel(a, Q, sh, ds, b, true, nmax, n_solve,
+                                       ds.bucket_count ? ds.bucket_count + 3 : nullptr, async, st, &kid);
             if (rc != RBFX_OK) return rc;
             hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);  // drop tuning failures
             for (int i = 0; i < 3; ++i) {
@@ -643,9 +769,9 @@ static int build_impl(const rbfx_build_args* a, bool async)
             }
             g_bucket_n[b] = async ? -1 : n_solve;
             g_bucket_kernel[b] = kid;
-            if (n_solve > 0) HIPCHK(launch_kernel_id(kid, Q, a->dim, true, nmax, n_solve, st));
+            if (n_solve > 0) HIPCHK(launch_kernel_id(kid, Q, sh, ds, b, true, nmax, n_solve, async, st));
         } else {
-            const int rc = solve_all(a, P, ds, nmax, true, async, st);
+            const int rc = solve_all(a, P, sh, ds, nmax, true, async, st);
             if (rc != RBFX_OK) return rc;
         }
         // FP32_REFINED: the FP32 solve is the main kernel (stage 1 stays empty)
@@ -736,8 +862,20 @@ int rbfx_last_build_counts(int64_t counts[16])
 int rbfx_tune_bucket(int32_t bucket, int32_t kernel)
 {
     if (bucket < 0 || bucket > 2) return set_err(RBFX_EINVAL, "bucket must be 0, 1 or 2");
-    if (kernel < -1 || kernel >= kNumKernels || (kernel >= 0 && !kernel_serves(kernel, bucket, true, 1)))
-        return set_err(RBFX_EINVAL, "kernel %d cannot serve bucket %d", kernel, bucket);
+    if (kernel < -1 || kernel >= kNumKernels)
+        return set_err(RBFX_EINVAL, "kernel %d is not a kernel id (0..%d)", kernel, kNumKernels - 1);
+    if (kernel >= 0) {
+        // a kernel that can serve the bucket for some configuration (a pinned kernel that
+        // cannot serve a particular build is ignored by that build)
+        Shape any{2, 20, 6, 1, RBFX_PHS3, 2, false};
+        bool ok = false;
+        for (int kmax : {20, 30, 40, 60})
+            for (int fp32 = 0; fp32 < 2; ++fp32) {
+                any.kmax = kmax;
+                ok = ok || kernel_serves(kernel, bucket, fp32 != 0, any, true);
+            }
+        if (!ok) return set_err(RBFX_EINVAL, "kernel %d cannot serve bucket %d", kernel, bucket);
+    }
     std::lock_guard<std::mutex> lk(g_mu);
     g_force_kernel[bucket] = kernel;
     g_err.clear();
@@ -750,6 +888,74 @@ int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3])
         n[b] = g_bucket_n[b];
         kernel[b] = g_bucket_kernel[b];
     }
+    return RBFX_OK;
+}
+
+int rbfx_last_build_stats(int64_t stats[8])
+{
+    for (int i = 0; i < 8; ++i) stats[i] = g_stats[i];
+    return RBFX_OK;
+}
+
+// Text format, one configuration per line:
+//   dim n_cols precision bucket kmax m basis_kind hermite kernel
+int rbfx_tune_save(const char* path)
+{
+    if (!path) return set_err(RBFX_EINVAL, "rbfx_tune_save: null path");
+    std::map<TuneKey, int> all;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        all = g_tune_saved;
+        for (const DeviceState& d : g_dev)
+            for (const auto& kv : d.tuned) all[kv.first] = kv.second;
+    }
+    FILE* f = std::fopen(path, "w");
+    if (!f) return set_err(RBFX_EINVAL, "rbfx_tune_save: cannot open %s for writing", path);
+    std::fprintf(f, "# rbfx autotuner choices: dim n_cols precision bucket kmax m basis hermite kernel\n");
+    for (const auto& kv : all) {
+        const TuneKey& k = kv.first;
+        std::fprintf(f, "%d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+                     std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k), std::get<7>(k), kv.second);
+    }
+    const bool ok = std::fclose(f) == 0;
+    if (!ok) return set_err(RBFX_EDEVICE, "rbfx_tune_save: write to %s failed", path);
+    g_err.clear();
+    return RBFX_OK;
+}
+
+int rbfx_tune_reset(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_tune_saved.clear();
+    for (DeviceState& d : g_dev) d.tuned.clear();
+    g_err.clear();
+    return RBFX_OK;
+}
+
+int rbfx_tune_load(const char* path)
+{
+    if (!path) return set_err(RBFX_EINVAL, "rbfx_tune_load: null path");
+    FILE* f = std::fopen(path, "r");
+    if (!f) return set_err(RBFX_EINVAL, "rbfx_tune_load: cannot open %s", path);
+    std::map<TuneKey, int> got;
+    char line[256];
+    int lineno = 0;
+    while (std::fgets(line, sizeof(line), f)) {
+        ++lineno;
+        if (line[0] == '#' || line[0] == '\n') continue;
+        int v[9];
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7],
+                        &v[8]) != 9 ||
+            v[8] < 0 || v[8] >= kNumKernels || v[3] < 0 || v[3] > 2) {
+            std::fclose(f);
+            return set_err(RBFX_EINVAL, "rbfx_tune_load: %s line %d is not a tuning entry", path, lineno);
+        }
+        got[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = v[8];
+    }
+    std::fclose(f);
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (const auto& kv : got) g_tune_saved[kv.first] = kv.second;
+    g_err.clear();
     return RBFX_OK;
 }
 

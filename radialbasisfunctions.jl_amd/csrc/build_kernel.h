@@ -391,15 +391,17 @@ __device__ __forceinline__ void axpy_row(Row& r, T a, const T* u, int n_cols)
 
 // Waves per SIMD the register allocation targets. The FP64 two-stencil kernels (up to
 // 4 right-hand sides) fit 3 waves (168 VGPRs) without spilling in 1-D and 2-D; in 3-D
-// they would spill at 3 waves and run at 2. No build kernel of two stencils per
-// wavefront may spill: with a spilling three-wave kernel (52-108 bytes per lane),
-// repeated identical builds of 1-D stencils (11-row systems, short-lived wavefronts)
-// differed from run to run in single stencils, up to wrong and singular results, on
-// the wavefronts that were the third on their compute unit; the spill-free kernels are
-// bit-identical in every repetition.
+// they would spill at 3 waves and run at 2. No build kernel may spill to scratch memory:
+// with a spilling three-wave kernel (52-108 bytes per lane), repeated identical builds
+// of 1-D stencils (11-row systems, short-lived wavefronts) differed from run to run in
+// single stencils, up to wrong and singular results, on the wavefronts that were the
+// third on their compute unit; the spill-free kernels are bit-identical in every
+// repetition. The one-stencil kernels (64-row register rows) need more than 256
+// registers: at one wave per SIMD the allocator keeps the overflow in accumulation
+// registers (AGPRs), so they too use no scratch memory.
 template <typename T, int NMAX, int DIM, int RC>
 struct BuildOccupancy {
-    static constexpr int waves = (sizeof(T) == 8 && NMAX <= 32 && RC <= 4 && DIM <= 2) ? 3 : 2;
+    static constexpr int waves = NMAX == 64 ? 1 : ((sizeof(T) == 8 && NMAX <= 32 && RC <= 4 && DIM <= 2) ? 3 : 2);
 };
 
 // ------------------------------------------------------------------ kernel
@@ -496,7 +498,10 @@ k_build(const BuildDev P)
     if (P.is_boundary) {
         const bool ib = is_pt && P.is_boundary[gidx];
         hermite = group_bits<NMAX>(__ballot(ib)) != 0ull;
-        if (alive && hermite && e < P.n_data && P.is_boundary[e]) {
+        // a Dirichlet eval node is an identity row whatever its neighbours are, the rule
+        // count_nonzeros sizes its row by (execution.jl:39-69), so the row never holds
+        // more than the one entry it was given
+        if (alive && e < P.n_data && P.is_boundary[e]) {
             const int bi = P.bnd_index[e];
             if (P.bc_alpha[bi] == 1.0 && P.bc_beta[bi] == 0.0) {
                 // identity row (fill_dirichlet_entry!, execution.jl:297-303)

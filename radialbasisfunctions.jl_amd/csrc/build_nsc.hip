@@ -1,0 +1,28 @@
+// Dispatch of the null-space Cholesky build over stencil shape, dimension and
+// operator columns (kernel template in build_nsc_kernel.h).
+#include "build_nsc.h"
+
+namespace rbfx {
+
+hipError_t launch_nsc_k20a(const BuildDev& P, int dim, int64_t n_work, hipStream_t s);
+hipError_t launch_nsc_k32(const BuildDev& P, int dim, int64_t n_work, hipStream_t s);
+
+// Built shapes (one operator column; 32 lanes per stencil, one row each): up to 20
+// points with up to 6 monomials (3 waves per SIMD), up to 32 points with up to 10
+// monomials (2 waves per SIMD). Both compile without spilling; shapes with more
+// columns or points, where the refinement step's working set spilled, are solved by the
+// rook-pivoted kernels.
+bool nsc_fits(int dim, int kmax, int m, int n_cols, int variant)
+{
+    if (variant != 0 || dim < 1 || dim > 3 || n_cols != 1 || kmax < 1 || m < 0 || m >= kmax) return false;
+    return kmax <= 32 && m <= 10;
+}
+
+hipError_t launch_build_nsc(const BuildDev& P, int dim, int kmax, int m, int variant, int64_t n_work, hipStream_t s)
+{
+    if (!nsc_fits(dim, kmax, m, P.n_cols, variant)) return hipErrorInvalidValue;
+    if (kmax <= 20 && m <= 6) return launch_nsc_k20a(P, dim, n_work, s);
+    return launch_nsc_k32(P, dim, n_work, s);
+}
+
+}  // namespace rbfx

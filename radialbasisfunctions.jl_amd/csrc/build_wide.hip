@@ -87,7 +87,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     }
     // stencil classification (types.jl:208-223)
     const bool hermite = P.is_boundary && (__ballot(ib[0] || ib[1]) != 0ull);
-    if (hermite && e < P.n_data && P.is_boundary[e]) {
+    // a Dirichlet eval node is an identity row whatever its neighbours are (the row
+    // length count_nonzeros gives it, execution.jl:39-69)
+    if (P.is_boundary && e < P.n_data && P.is_boundary[e]) {
         const int bi = P.bnd_index[e];
         if (P.bc_alpha[bi] == 1.0 && P.bc_beta[bi] == 0.0) {
             // identity row (fill_dirichlet_entry!, execution.jl:297-303)

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "knn.h"
+#include "knn_grid.h"
 
 namespace rbfx {
 
@@ -225,29 +226,10 @@ static hipError_t knn_dim(KnnWorkspace& ws, const double* data, int64_t n_data, 
 {
     Grid G;
     G.dim = DIM;
-    double vol = 1.0, ext[3] = {0, 0, 0};
-    for (int d = 0; d < DIM; ++d) {
-        ext[d] = bb[3 + d] - bb[d];
-        if (!(ext[d] > 0)) ext[d] = 1e-12;
-        vol *= ext[d];
-    }
-    // about 3 points per cell
-    const double target = 3.0;
-    double s = pow(vol * target / (double)n_data, 1.0 / DIM);
-    int64_t ncell = 1;
-    for (int d = 0; d < DIM; ++d) {
-        double nd = floor(ext[d] / s) + 1;
-        if (nd > 1e6) nd = 1e6;
-        G.n[d] = (int)nd;
-    }
-    for (int d = DIM; d < 3; ++d) G.n[d] = 1;
-    // guard against degenerate extents blowing up the cell count
-    for (;;) {
-        ncell = (int64_t)G.n[0] * G.n[1] * G.n[2];
-        if (ncell <= 4 * n_data + 16) break;
-        s *= 1.25;
-        for (int d = 0; d < DIM; ++d) G.n[d] = (int)(floor(ext[d] / s) + 1);
-    }
+    double ext[3] = {0, 0, 0};
+    for (int d = 0; d < DIM; ++d) ext[d] = bb[3 + d] - bb[d];
+    double s = 1.0;
+    const int64_t ncell = knn_grid_size(DIM, ext, n_data, &s, G.n);
     G.s = s;
     G.inv_s = 1.0 / s;
     for (int d = 0; d < 3; ++d) G.lo[d] = (d < DIM) ? bb[d] : 0.0;

@@ -25,7 +25,21 @@ G = {
             {"basis": "IMQ", "eps": 2.0, "d1": 4 / (21 * s21), "grad": [4 / (21 * s21), 8 / (21 * s21)],
              "d2": -4 / (49 * s21), "src": "test/basis/inverse_multiquadric.jl:40-42"},
             {"basis": "Gaussian", "eps": 2.0, "d1": 8 / math.exp(20), "grad": [8 / math.exp(20), 16 / math.exp(20)],
-             "d2": 56 / math.exp(20), "src": "test/basis/gaussian.jl:98-100"},
+             "d2": 56 / math.exp(20), "src": "test/basis/gaussian.jl:40-42"},
+        ],
+    },
+    # phi(x1, x2) at x1 = (1, 2), x2 = (2, 4), r = sqrt(5): the closed forms the
+    # reference's "Distances" testsets assert
+    "basis_values": {
+        "x1": [1.0, 2.0],
+        "x2": [2.0, 4.0],
+        "cases": [
+            {"basis": "PHS1", "value": s5, "src": "test/basis/polyharmonic_spline.jl:33-35"},
+            {"basis": "PHS3", "value": 5 * s5, "src": "test/basis/polyharmonic_spline.jl:100-102"},
+            {"basis": "PHS5", "value": 25 * s5, "src": "test/basis/polyharmonic_spline.jl:167-169"},
+            {"basis": "PHS7", "value": 125 * s5, "src": "test/basis/polyharmonic_spline.jl:233-235"},
+            {"basis": "IMQ", "eps": 2.0, "value": 1 / s21, "src": "test/basis/inverse_multiquadric.jl:29-31"},
+            {"basis": "Gaussian", "eps": 2.0, "value": math.exp(-20), "src": "test/basis/gaussian.jl:29-31"},
         ],
     },
     "monomials": [

@@ -47,6 +47,12 @@ def test_operator_lowering(RBF):
     assert n == 3 and t == [(c, a.OP_D1, c, 0, 1.0) for c in range(3)]
     n, t = RBF.Hessian(2).lower(2)
     assert n == 3 and [x[:4] for x in t] == [(0, a.OP_D2, 0, 0), (1, a.OP_MIXED, 0, 1), (2, a.OP_D2, 1, 1)]
+    # D(D+1)/2 upper-triangular columns in _kth_sym_pair order (hessian.jl:16-22,
+    # operators.jl:193-209): 6 columns in 3-D, within RBFX_MAX_COLS
+    n, t = RBF.Hessian(3).lower(3)
+    pairs = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    assert n == 6 <= a.MAX_COLS
+    assert [x[:4] for x in t] == [(c, a.OP_D2 if i == j else a.OP_MIXED, i, j) for c, (i, j) in enumerate(pairs)]
 
 
 def test_constructor_errors(RBF):

@@ -30,6 +30,20 @@ def test_basis_derivative_goldens(O):
             assert O.phi(b, x1, x2) == pytest.approx(r ** int(c["basis"][3]), rel=1e-14)
 
 
+def test_basis_value_goldens(O):
+    # phi itself for every basis, at the points of the reference's "Distances" testsets
+    # (PHS r^p, IMQ 1/sqrt((eps r)^2 + 1) = 1/sqrt(21), Gaussian exp(-(eps r)^2) = e^-20)
+    g = GOLD["basis_values"]
+    for c in g["cases"]:
+        b = _basis(O, c["basis"], c.get("eps", 1.0))
+        assert O.phi(b, g["x1"], g["x2"]) == pytest.approx(c["value"], rel=1e-14), c["src"]
+    # the shape parameter enters as (eps r)^2, not eps r^2: at eps = 2 the two differ
+    imq = _basis(O, "IMQ", 2.0)
+    assert O.phi(imq, [0.0, 0.0], [0.3, 0.4]) == pytest.approx(1.0 / math.sqrt(2.0), rel=1e-15)
+    gau = _basis(O, "Gaussian", 2.0)
+    assert O.phi(gau, [0.0, 0.0], [0.3, 0.4]) == pytest.approx(math.exp(-1.0), rel=1e-15)
+
+
 def _fd_grad(f, x, h=1e-6):
     x = np.asarray(x, float)
     g = np.zeros_like(x)
@@ -51,6 +65,53 @@ def test_hessian_matches_finite_differences(O, name):
     assert np.allclose(H, H.T, rtol=1e-12)
     # mixed partial = Hessian entry (basis.jl:87-92)
     assert O.lrbf(b, O.abi.OP_MIXED, x1, x2, 0, 1) == pytest.approx(H[0, 1], rel=1e-13)
+
+
+@pytest.mark.parametrize("name", ["PHS1", "PHS3", "PHS5", "PHS7", "IMQ", "Gaussian"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_gradient_is_derivative_of_phi(O, name, dim):
+    # links phi (pinned by its closed form above) to the derivative code (pinned by the
+    # reference's derivative goldens): a wrong phi or a wrong derivative fails here
+    b = _basis(O, name, 1.7 if name in ("IMQ", "Gaussian") else 1.0)
+    x1 = np.array([0.3, -0.2, 0.5])[:dim]
+    x2 = np.array([-0.1, 0.4, 0.2])[:dim]
+    fd = _fd_grad(lambda x: O.phi(b, x, x2), x1)
+    assert np.allclose(O.grad(b, x1, x2), fd, rtol=1e-7, atol=1e-9)
+    for d in range(dim):
+        assert O.lrbf(b, O.abi.OP_D1, x1, x2, d) == pytest.approx(fd[d], rel=1e-7, abs=1e-9)
+
+
+@pytest.mark.parametrize("name", ["IMQ", "Gaussian"])
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_imq_gaussian_laplacian(O, name, dim):
+    # inverse_multiquadric.jl:77-83 and gaussian.jl:160-163 sum d^2/dx_d^2 over the
+    # dimensions, so (unlike PHS) the Laplacian is the true one in every dimension:
+    # it equals a finite-difference Laplacian of phi and the trace of the Hessian,
+    # and each d^2 term equals the Hessian diagonal
+    for eps in (0.8, 2.5):
+        b = _basis(O, name, eps)
+        x1 = np.array([0.3, -0.2, 0.5])[:dim]
+        x2 = np.array([-0.1, 0.4, 0.2])[:dim]
+        h = 1e-4
+        lap_fd = 0.0
+        for d in range(dim):
+            e = np.zeros(dim)
+            e[d] = h
+            lap_fd += (O.phi(b, x1 + e, x2) - 2 * O.phi(b, x1, x2) + O.phi(b, x1 - e, x2)) / h ** 2
+        lap = O.lrbf(b, O.abi.OP_LAP, x1, x2)
+        assert lap == pytest.approx(lap_fd, rel=1e-5)
+        H = O.hess(b, x1, x2)
+        assert lap == pytest.approx(np.trace(H), rel=1e-13)
+        for d in range(dim):
+            assert O.lrbf(b, O.abi.OP_D2, x1, x2, d) == pytest.approx(H[d, d], rel=1e-13, abs=1e-15)
+
+
+def test_phs_laplacian_is_hessian_trace_in_3d(O):
+    # in 3-D the reference's PHS constants are the true Laplacian: lap = trace(H)
+    for name in ("PHS1", "PHS3", "PHS5", "PHS7"):
+        b = _basis(O, name)
+        x1, x2 = np.array([0.3, -0.2, 0.5]), np.array([-0.1, 0.4, 0.2])
+        assert O.lrbf(b, O.abi.OP_LAP, x1, x2) == pytest.approx(np.trace(O.hess(b, x1, x2)), rel=1e-13)
 
 
 @pytest.mark.parametrize("name", ["PHS1", "PHS3", "PHS5", "PHS7"])
@@ -323,6 +384,61 @@ def test_laplacian_end_to_end_manufactured(O):
     res = O.spmv(ptr, col, val[0], uu)
     assert np.abs(res[~isb] - 4.0).max() < 1e-10
     assert np.abs(res[isb] - ub).max() < 1e-10  # boundary rows apply the BC operator
+
+
+def test_assembled_system_goldens(O):
+    # _build_collocation_matrix! on three points, PHS3 with degree 1 (test/solve.jl:7-31):
+    # A[i,j] = r_ij^3 in the RBF block and rows [1, x, y] in the polynomial block, with
+    # the m x m zero block; the system condition_numbers() measures is this matrix
+    import ctypes as C
+
+    g = GOLD["collocation_small"]
+    pts = np.array(g["points"])
+    ev = np.array([g["eval_point"]])
+    op = O.abi.make_operator(1, [(0, O.abi.OP_D1, 0, 0, 1.0)])
+    a, keep = O.build(pts, ev, np.arange(3, dtype=np.int32)[None, :], O.basis_struct(3, 1), op, _want_args=True)
+    A = np.zeros(36)
+    n = O.lib().orc_assemble(C.byref(a), 0, 0, A.ctypes.data_as(C.POINTER(C.c_double)))
+    assert n == 6
+    A = A.reshape(6, 6)
+    for i in range(3):
+        for j in range(3):
+            assert A[i, j] == pytest.approx(math.dist(pts[i], pts[j]) ** 3, rel=1e-14, abs=0), g["src"]
+        assert np.allclose(A[i, 3:], [1.0, pts[i, 0], pts[i, 1]]), g["src"]
+        assert np.allclose(A[3:, i], [1.0, pts[i, 0], pts[i, 1]]), g["src"]
+    assert not A[3:, 3:].any()
+    kap = O.condition_numbers(pts, ev, np.arange(3, dtype=np.int32)[None, :], O.basis_struct(3, 1), op,
+                              scaled=False)
+    s = np.linalg.svd(A, compute_uv=False)
+    assert kap[0] == pytest.approx(s[0] / s[-1], rel=1e-10)
+
+
+def _dirichlet_adjl_without_boundary(O, pts, isb):
+    """adjl where each Dirichlet eval node's stencil holds only interior points, the
+    other rows are the 10 nearest neighbours (which include boundary points)."""
+    adj = O.knn(pts, pts, 10)
+    inner = np.nonzero(~isb)[0][:10].astype(np.int32)
+    for i in np.nonzero(isb)[0]:
+        adj[i] = inner
+    return adj
+
+
+def test_dirichlet_eval_row_without_boundary_neighbours(O):
+    # count_nonzeros gives a Dirichlet eval node one entry whatever its stencil holds;
+    # the build writes its identity row there and leaves every other row intact
+    H = GOLD["hermite_domain_12"]
+    pts = np.array(H["points"])
+    isb = np.array(H["is_boundary"])
+    adj = _dirichlet_adjl_without_boundary(O, pts, isb)
+    herm = (isb, np.array([1.0, 1.0]), np.array([0.0, 0.0]), np.array(H["normals"]))
+    ptr, col, val, st, rc = O.build(pts, pts, adj, O.basis_struct(3, 2), _lap_op(O), hermite=herm)
+    assert rc == 0
+    for i in np.nonzero(isb)[0]:
+        assert ptr[i + 1] - ptr[i] == 1 and col[ptr[i]] == i and val[0, ptr[i]] == 1.0 and st[i] == 1
+    u = pts[:, 1] ** 2
+    res = O.spmv(ptr, col, val[0], u)
+    assert np.abs(res[~isb] - 2.0).max() < 1e-10
+    assert np.allclose(res[isb], u[isb])
 
 
 def test_count_nonzeros_dirichlet_rows(O):
