@@ -49,9 +49,10 @@ int set_err(int code, const char* fmt, ...)
     } while (0)
 
 // Autotuner configuration key: dimension, operator columns, precision mode, size
-// bucket, largest stencil (k points, m monomials), basis kind and whether the build has
-// boundary data.
-typedef std::tuple<int, int, int, int, int, int, int, int> TuneKey;
+// bucket, largest stencil (k points, m monomials), basis kind, whether the build has
+// boundary data and whether every stencil has the same shape (no ragged k, no degree
+// map: the null-space kernels serve only such builds).
+typedef std::tuple<int, int, int, int, int, int, int, int, int> TuneKey;
 
 // per-device library scratch (grow-only)
 struct DeviceState {
@@ -266,6 +267,7 @@ constexpr int32_t kTuneSample = 16384;
 struct Shape {
     int dim, kmax, m, n_cols, basis_kind, poly_deg;
     bool hermite;
+    bool uniform;  // every stencil has k points and m monomials (no ragged k, no degree map)
 };
 
 static bool is_nsc(int kid) { return kid == K_NSC || kid == K_NSC2; }
@@ -285,7 +287,7 @@ static bool kernel_serves(int kid, int bucket, bool fp32, const Shape& sh, bool 
     case K_REG64_F64: return fp32 && bucket <= 1;
     case K_NSC:
     case K_NSC2: {
-        if (rbfx::nsc_sign(sh.basis_kind, sh.poly_deg) == 0) return false;
+        if (!sh.uniform || rbfx::nsc_sign(sh.basis_kind, sh.poly_deg) == 0) return false;
         const int cap = bucket == 0 ? 32 : (bucket == 1 ? 64 : RBFX_MAX_N);
         const int kb = std::min(sh.kmax, cap);
         return rbfx::nsc_fits(sh.dim, kb, sh.m, sh.n_cols, kid == K_NSC2 ? 1 : 0);
@@ -353,8 +355,14 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const 
         *kid = forced;
         return RBFX_OK;
     }
-    const TuneKey key{a->dim, sh.n_cols, a->precision, bucket, sh.kmax, sh.m, sh.basis_kind, sh.hermite ? 1 : 0};
+    const TuneKey key{a->dim,         sh.n_cols, a->precision, bucket, sh.kmax, sh.m, sh.basis_kind, sh.hermite ? 1 : 0,
+                      sh.uniform ? 1 : 0};
+    // a cached or loaded choice is used only where that kernel serves this build
     auto it = ds.tuned.find(key);
+    if (it != ds.tuned.end() && !kernel_serves(it->second, bucket, fp32, sh, false)) {
+        ds.tuned.erase(it);
+        it = ds.tuned.end();
+    }
     if (it == ds.tuned.end()) {
         auto jt = g_tune_saved.find(key);  // a map loaded with rbfx_tune_load
         if (jt != g_tune_saved.end() && kernel_serves(jt->second, bucket, fp32, sh, false)) {
@@ -678,6 +686,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
     sh.basis_kind = kind;
     sh.poly_deg = deg;
     sh.hermite = a->is_boundary != nullptr;
+    sh.uniform = !a->adj_ptr && !a->poly_deg_map;
     // stage boundaries: ev0 | main kernel | ev1 | FP32 list solve | ev2 | FP64 fallback | ev3
     // (blocking builds only; an asynchronous build records nothing but kernels)
     auto mark = [&](int i) -> hipError_t {
@@ -867,7 +876,7 @@ int rbfx_tune_bucket(int32_t bucket, int32_t kernel)
     if (kernel >= 0) {
         // a kernel that can serve the bucket for some configuration (a pinned kernel that
         // cannot serve a particular build is ignored by that build)
-        Shape any{2, 20, 6, 1, RBFX_PHS3, 2, false};
+        Shape any{2, 20, 6, 1, RBFX_PHS3, 2, false, true};
         bool ok = false;
         for (int kmax : {20, 30, 40, 60})
             for (int fp32 = 0; fp32 < 2; ++fp32) {
@@ -898,7 +907,7 @@ int rbfx_last_build_stats(int64_t stats[8])
 }
 
 // Text format, one configuration per line:
-//   dim n_cols precision bucket kmax m basis_kind hermite kernel
+//   dim n_cols precision bucket kmax m basis_kind hermite uniform kernel
 int rbfx_tune_save(const char* path)
 {
     if (!path) return set_err(RBFX_EINVAL, "rbfx_tune_save: null path");
@@ -911,11 +920,12 @@ int rbfx_tune_save(const char* path)
     }
     FILE* f = std::fopen(path, "w");
     if (!f) return set_err(RBFX_EINVAL, "rbfx_tune_save: cannot open %s for writing", path);
-    std::fprintf(f, "# rbfx autotuner choices: dim n_cols precision bucket kmax m basis hermite kernel\n");
+    std::fprintf(f, "# rbfx autotuner choices: dim n_cols precision bucket kmax m basis hermite uniform kernel\n");
     for (const auto& kv : all) {
         const TuneKey& k = kv.first;
-        std::fprintf(f, "%d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
-                     std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k), std::get<7>(k), kv.second);
+        std::fprintf(f, "%d %d %d %d %d %d %d %d %d %d\n", std::get<0>(k), std::get<1>(k), std::get<2>(k),
+                     std::get<3>(k), std::get<4>(k), std::get<5>(k), std::get<6>(k), std::get<7>(k), std::get<8>(k),
+                     kv.second);
     }
     const bool ok = std::fclose(f) == 0;
     if (!ok) return set_err(RBFX_EDEVICE, "rbfx_tune_save: write to %s failed", path);
@@ -943,14 +953,14 @@ int rbfx_tune_load(const char* path)
     while (std::fgets(line, sizeof(line), f)) {
         ++lineno;
         if (line[0] == '#' || line[0] == '\n') continue;
-        int v[9];
-        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7],
-                        &v[8]) != 9 ||
-            v[8] < 0 || v[8] >= kNumKernels || v[3] < 0 || v[3] > 2) {
+        int v[10];
+        if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7],
+                        &v[8], &v[9]) != 10 ||
+            v[9] < 0 || v[9] >= kNumKernels || v[3] < 0 || v[3] > 2) {
             std::fclose(f);
             return set_err(RBFX_EINVAL, "rbfx_tune_load: %s line %d is not a tuning entry", path, lineno);
         }
-        got[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7]}] = v[8];
+        got[TuneKey{v[0], v[1], v[2], v[3], v[4], v[5], v[6], v[7], v[8]}] = v[9];
     }
     std::fclose(f);
     std::lock_guard<std::mutex> lk(g_mu);

@@ -30,7 +30,9 @@ __host__ __device__ inline int nsc_sign(int basis_kind, int poly_deg)
 // column, up to 32 points and 10 monomials; no other variant is built.
 bool nsc_fits(int dim, int kmax, int m, int n_cols, int variant);
 
-// FP64 null-space build of the work (P.list / P.order / all stencils). Stencils whose
+// FP64 null-space build of the work (P.list / P.order / all stencils) of a uniform build
+// (no adj_ptr, no degree map: every stencil has P.k points and P.n_mono monomials, with
+// kmax = P.k and m = P.n_mono within nsc_fits). Stencils whose
 // factorisation breaks down, whose refinement does not settle, or that the kernel does
 // not serve (Hermite stencils are never passed here) are appended to P.fail_list for
 // the rook-pivoted kernels; every other stencil gets status P.ok_status.

@@ -21,6 +21,8 @@ bool nsc_fits(int dim, int kmax, int m, int n_cols, int variant)
 hipError_t launch_build_nsc(const BuildDev& P, int dim, int kmax, int m, int variant, int64_t n_work, hipStream_t s)
 {
     if (!nsc_fits(dim, kmax, m, P.n_cols, variant)) return hipErrorInvalidValue;
+    // uniform builds only: the kernel takes every stencil's k and m from P.k and P.n_mono
+    if (P.adj_ptr || P.deg_map || P.k != kmax || P.n_mono != m) return hipErrorInvalidValue;
     if (kmax <= 20 && m <= 6) return launch_nsc_k20a(P, dim, n_work, s);
     return launch_nsc_k32(P, dim, n_work, s);
 }

@@ -69,16 +69,47 @@ __device__ __forceinline__ double gmax(double v)
     return v;
 }
 
+// Square root, reciprocal square root and reciprocal in about one ulp: the hardware
+// estimates (v_rsq_f64, v_rcp_f64) refined by Newton steps. Their rounding differs from
+// the correctly rounded operations by at most an ulp or so, a relative perturbation of
+// the assembled system of order 1e-16, inside the conditioning bound the build meets.
+// nsc_sqrt(0) = 0 (the PHS diagonal); arguments are finite and not negative.
+__device__ __forceinline__ double nsc_rsq(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+    const double hx = 0.5 * x;
+    y = y * fma(-hx * y, y, 1.5);
+    y = y * fma(-hx * y, y, 1.5);
+    return y;
+}
+__device__ __forceinline__ double nsc_sqrt(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    double s = x * y, h = 0.5 * y;
+    const double r = fma(-s, h, 0.5);
+    s = fma(s, r, s);
+    h = fma(h, r, h);
+    s = fma(fma(-s, s, x), h, s);
+    return x > 0.0 ? s : 0.0;
+}
+__device__ __forceinline__ double nsc_rcp(double d)
+{
+    double y = __builtin_amdgcn_rcp(d);
+    y = fma(y, fma(-d, y, 1.0), y);
+    y = fma(y, fma(-d, y, 1.0), y);
+    return y;
+}
+
 // phi(r^2) with the basis kind a compile-time constant (the assembly and residual
 // loops are instantiated once per kind, so no kind switch sits inside them)
 template <int KIND>
 __device__ __forceinline__ double phi_kind(double eps, double r2)
 {
-    if constexpr (KIND == RBFX_PHS1) return sqrt(r2);
-    else if constexpr (KIND == RBFX_PHS3) return r2 * sqrt(r2);
-    else if constexpr (KIND == RBFX_PHS5) return (r2 * r2) * sqrt(r2);
-    else if constexpr (KIND == RBFX_PHS7) return (r2 * r2 * r2) * sqrt(r2);
-    else if constexpr (KIND == RBFX_IMQ) return 1.0 / sqrt((eps * eps) * r2 + 1.0);
+    if constexpr (KIND == RBFX_PHS1) return nsc_sqrt(r2);
+    else if constexpr (KIND == RBFX_PHS3) return r2 * nsc_sqrt(r2);
+    else if constexpr (KIND == RBFX_PHS5) return (r2 * r2) * nsc_sqrt(r2);
+    else if constexpr (KIND == RBFX_PHS7) return (r2 * r2 * r2) * nsc_sqrt(r2);
+    else if constexpr (KIND == RBFX_IMQ) return nsc_rsq((eps * eps) * r2 + 1.0);
     else return exp(-(eps * eps) * r2);
 }
 
@@ -134,32 +165,32 @@ k_nsc(const BuildDev P)
     for (int i = li; i < (int)(sizeof(Lds) / sizeof(double)); i += GS) reinterpret_cast<double*>(&S)[i] = 0.0;
     const int64_t n_work = P.list ? (int64_t)*P.list_len : P.n_eval;
     const int64_t wi = lb * G + g;
-    bool alive = wi < n_work;
+    // (long-lived per-lane flags are ints: as bools they would be held as lane masks in
+    // scalar registers, which run out and spill)
+    int alive = wi < n_work;
     int64_t e = alive ? (P.list ? (int64_t)P.list[wi] : (P.order ? (int64_t)P.order[wi] : wi)) : 0;
     if (e < 0 || e >= P.n_eval) {
-        alive = false;
+        alive = 0;
         e = 0;
     }
-    const int64_t s0 = alive ? (P.adj_ptr ? P.adj_ptr[e] : e * (int64_t)P.k) : 0;
-    const int kk = alive ? (P.adj_ptr ? (int)(P.adj_ptr[e + 1] - s0) : P.k) : 0;
-    const int64_t o0 = alive ? (P.out_ptr ? P.out_ptr[e] : e * (int64_t)P.k) : 0;
-    const int m_e = alive ? stencil_n_mono(P, e, DIM) : 0;
-    const int M = m_e < 0 ? 0 : m_e;
-    if (alive && m_e < 0) {
-        if (li == 0 && P.status) P.status[e] = RBFX_ST_INVALID;
-        alive = false;
+    // Uniform builds only (fixed k, one polynomial degree; the host checks it, and that
+    // k and m fit the instantiation): the stencil shape is a kernel argument, so every
+    // condition on it is a scalar branch. A group without work (past the end of the
+    // launch, or handed back) computes on zeros, harmlessly, and writes nothing.
+    const int kk = P.k;
+    const int M = P.n_mono;
+    const int m_wave = M;
+    int handoff = 0;
+    int fail = 0;
+    if (P.adj_ptr || P.deg_map) {  // (the launcher refuses such builds; nothing is read)
+        handoff = alive;
+        alive = 0;
     }
-    // stencils outside this kernel's shape go to the rook kernels untouched
-    bool handoff = alive && (kk < 1 || kk > KMAX || M > MMAX || M >= kk);
-    bool fail = false;
-    if (handoff) alive = false;
-    int m_wave = alive ? M : 0;
-#pragma unroll
-    for (int x = GS; x < 64; x <<= 1) m_wave = max(m_wave, __shfl_xor(m_wave, x, 64));
-    m_wave = __builtin_amdgcn_readfirstlane(m_wave);
+    const int64_t o0 = alive ? (P.out_ptr ? P.out_ptr[e] : e * (int64_t)kk) : 0;
+    const int64_t s0 = e * (int64_t)kk;
 
     // ---- this lane's points (rows li + s*GS)
-    bool is_pt[SL];
+    int is_pt[SL];
     double xi[SL][DIM];
     int gidx[SL];
     double xe[DIM];
@@ -169,7 +200,7 @@ k_nsc(const BuildDev P)
 #pragma unroll
     for (int s = 0; s < SL; ++s) {
         const int row = li + s * GS;
-        is_pt[s] = alive && row < kk;
+        is_pt[s] = alive && row < kk;  // (for the gather; the arithmetic uses row < kk)
         gidx[s] = 0;
         double r2 = 0.0;
 #pragma unroll
@@ -195,12 +226,13 @@ k_nsc(const BuildDev P)
 #pragma unroll
         for (int s = 0; s < SL; ++s) ib = ib || (is_pt[s] && P.is_boundary[gidx[s]]);
         if (alive && group_bits<GS>(__ballot(ib)) != 0ull) {
-            handoff = true;
-            alive = false;
+            handoff = 1;
+            alive = 0;
         }
-#pragma unroll
-        for (int s = 0; s < SL; ++s) is_pt[s] = is_pt[s] && alive;
     }
+    // point rows of the arithmetic: the same lanes in every group
+#pragma unroll
+    for (int s = 0; s < SL; ++s) is_pt[s] = li + s * GS < kk;
     double h = sqrt(gmax<GS>(r2max));
     if (!(h > 0.0)) h = 1.0;
     const double hinv = 1.0 / h;
@@ -266,7 +298,7 @@ k_nsc(const BuildDev P)
             double ga[RC];
 #pragma unroll
             for (int c = 0; c < RC; ++c) ga[c] = 0.0;
-            if (alive && a < M) {
+            if (a < M) {
                 const double z[DIM] = {};
                 for (int t = 0; t < n_terms; ++t) {
                     const TermDev tm = s_terms[t];
@@ -290,7 +322,10 @@ k_nsc(const BuildDev P)
             p[s][a] = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
     __syncthreads();
 
-    // ---- collocation rows Phi[row][j] = phi(|xi_row - xi_j|), zero outside the stencil
+    // ---- collocation rows Phi[row][j] = phi(|xi_row - xi_j|). Rows and columns past the
+    // stencil (k <= j < KMAX) hold phi of zero offsets: finite values that never reach a
+    // stencil entry, since every reflector is zero there and the LDL^T and the solves
+    // stop at row and column k.
     double phi[SL][KMAX];
 #define RBFX_NSC_ASSEMBLE(BK)                                                            \
     _Pragma("unroll") for (int j = 0; j < KMAX; ++j)                                     \
@@ -305,8 +340,7 @@ k_nsc(const BuildDev P)
                 const double t_ = xi[s][d] - xj[d];                                      \
                 q2 += t_ * t_;                                                           \
             }                                                                            \
-            const double v_ = phi_kind<BK>(eps_h, q2);                                   \
-            phi[s][j] = (is_pt[s] && j < kk) ? v_ : 0.0;                                 \
+            phi[s][j] = phi_kind<BK>(eps_h, q2);                                         \
         }                                                                                \
     }
     switch (kind) {
@@ -323,7 +357,7 @@ k_nsc(const BuildDev P)
 #pragma unroll
     for (int c = 0; c < MMAX; ++c) {
         if (c < m_wave) {
-            const bool act = alive && c < M;
+            const bool act = c < M;
             double a1 = 0.0, a2 = 0.0;
 #pragma unroll
             for (int s = 0; s < SL; ++s) {
@@ -336,8 +370,8 @@ k_nsc(const BuildDev P)
             const double sq = sqrt(nrm2);
             const double alpha = xc >= 0.0 ? -sq : sq;
             const double den = nrm2 - alpha * xc;  // |v|^2 / 2
-            if (act && !(den > 0.0 && den < 1e300)) fail = true;  // P rank-deficient
-            const double tau = (act && den > 0.0) ? 1.0 / den : 0.0;
+            if (act && !(den > 0.0 && den < 1e300)) fail = 1;  // P rank-deficient
+            const double tau = (act && den > 0.0) ? nsc_rcp(den) : 0.0;
             double v[SL];
 #pragma unroll
             for (int s = 0; s < SL; ++s) {
@@ -428,8 +462,8 @@ k_nsc(const BuildDev P)
     for (int s = 0; s < SL; ++s) dreg[s] = 1.0;
 #pragma unroll
     for (int j = 0; j < KMAX; ++j) {
-        const bool step = alive && !fail && j >= M && j < kk;
-        if (__ballot(step) == 0ull) continue;
+        const bool step = j >= M && j < kk;
+        if (!step) continue;
         const int js = j / GS, jl = j % GS;
         if (step && li == jl) {
 #pragma unroll
@@ -439,8 +473,8 @@ k_nsc(const BuildDev P)
         __syncthreads();
         const double d = S.piv[j];
         const bool bad = step && !(d > 0.0 && d < 1e300);
-        if (bad) fail = true;
-        const double dinv = step && !bad ? 1.0 / d : 0.0;
+        if (bad) fail = 1;
+        const double dinv = step && !bad ? nsc_rcp(d) : 0.0;
         if (li == jl) dreg[js] = step && !bad ? d : 1.0;
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
@@ -487,7 +521,7 @@ k_nsc(const BuildDev P)
             const int as = a / GS, al = a % GS;
             if (a < m_wave) {
                 if (li == al) {
-                    S.w1[a] = (alive && a < M) ? (S.g[cc][a] - acc[as]) / S.R[a][a] : 0.0;
+                    S.w1[a] = a < M ? (S.g[cc][a] - acc[as]) * nsc_rcp(S.R[a][a]) : 0.0;
                     S.btm[cc][a] = bt[as];
                 }
                 __syncthreads();
@@ -515,8 +549,8 @@ k_nsc(const BuildDev P)
         // forward substitution L y = t: row j, once final, goes to the rows below it
 #pragma unroll
         for (int j = 0; j < KMAX; ++j) {
-            const bool step = alive && j >= M && j < kk;
-            if (__ballot(step) == 0ull) continue;
+            const bool step = j >= M && j < kk;
+            if (!step) continue;
             const int js = j / GS, jl = j % GS;
             if (step && li == jl) S.piv[KMAX] = t[js];
             __syncthreads();
@@ -530,11 +564,11 @@ k_nsc(const BuildDev P)
         }
         // D z = y, then L^T u = z: row i, once final, sends l_ij u_i to every row j < i
 #pragma unroll
-        for (int s = 0; s < SL; ++s) t[s] = t[s] / dreg[s];
+        for (int s = 0; s < SL; ++s) t[s] = t[s] * nsc_rcp(dreg[s]);
 #pragma unroll
         for (int i = KMAX - 1; i >= 1; --i) {
-            const bool step = alive && i > M && i < kk;
-            if (__ballot(step) == 0ull) continue;
+            const bool step = i > M && i < kk;
+            if (!step) continue;
             const int is = i / GS, il = i % GS;
             if (step && li == il) {
 #pragma unroll
@@ -554,7 +588,7 @@ k_nsc(const BuildDev P)
 #pragma unroll
         for (int s = 0; s < SL; ++s) {
             const int row = li + s * GS;
-            const bool pt = alive && row < kk;
+            const bool pt = row < kk;
             const double w1v = S.w1[row < MMAX ? row : 0];
             w[s] = !pt ? 0.0 : (row < M ? w1v : t[s]);
             S.wt[cc][row] = w[s];
@@ -606,7 +640,7 @@ k_nsc(const BuildDev P)
         for (int a = MMAX - 1; a >= 0; --a) {
             const int as = a / GS, al = a % GS;
             if (a < m_wave) {
-                if (li == al) out[cc][a] = (alive && a < M) ? (q[as] - acc[as]) / S.R[a][a] : 0.0;
+                if (li == al) out[cc][a] = a < M ? (q[as] - acc[as]) * nsc_rcp(S.R[a][a]) : 0.0;
                 __syncthreads();
 #pragma unroll
                 for (int s = 0; s < SL; ++s) {
@@ -664,7 +698,7 @@ k_nsc(const BuildDev P)
                 const double t_ = xi[s][d] - xj[d];                                      \
                 q2 += t_ * t_;                                                           \
             }                                                                            \
-            const double v_ = (is_pt[s] && j < kk) ? phi_kind<BK>(eps_h, q2) : 0.0;      \
+            const double v_ = phi_kind<BK>(eps_h, q2);                                   \
             _Pragma("unroll") for (int c = 0; c < RC; ++c) r[s][c] = fma(-v_, S.wsol[c][j], r[s][c]); \
         }                                                                                \
         __builtin_amdgcn_sched_barrier(0);                                               \
@@ -693,7 +727,7 @@ k_nsc(const BuildDev P)
 #pragma unroll
                         for (int s = 0; s < SL; ++s) loc = fma(pa[s], S.wsol[c][li + s * GS], loc);
                         const double ptw = gsum<GS>(loc);
-                        if (li == a % GS) S.g[c][a] = (alive && a < M) ? S.g[c][a] - ptw : 0.0;
+                        if (li == a % GS) S.g[c][a] = a < M ? S.g[c][a] - ptw : 0.0;
                     }
                 }
             }
@@ -721,7 +755,7 @@ k_nsc(const BuildDev P)
         }
         dmax = gmax<GS>(dmax);
         wmax = gmax<GS>(wmax);
-        if (alive && !(dmax <= 1e-7 * wmax)) fail = true;
+        if (!(dmax <= 1e-7 * wmax)) fail = 1;
     }
 
     // ---- outputs

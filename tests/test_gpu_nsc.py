@@ -165,10 +165,50 @@ def test_tuning_map_saved_and_reused(RBF, gpu, tmp_path):
     assert tuned > 0
     RBF._lib.tune_save(path)
     lines = [l for l in path.read_text().splitlines() if not l.startswith("#")]
-    assert lines and all(len(l.split()) == 9 for l in lines)
+    assert lines and all(len(l.split()) == 10 for l in lines)
     RBF._lib.tune_reset()
     RBF._lib.tune_load(path)
     RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20)
     assert RBF._lib.last_build_stats()[1] == 0
     assert RBF._lib.last_build_buckets()[1][0] == kern
+    RBF._lib.tune_reset()
+
+
+def test_tuned_choice_never_given_to_ragged_build(RBF, O, gpu, tmp_path):
+    """The null-space kernel takes k and m from the build, so it serves only builds whose
+    stencils all have one shape. A choice the autotuner made for a uniform build (k = 30,
+    bucket 33..64 rows), or a loaded map that names it for a ragged build of the same
+    largest shape, is not used for a ragged adjacency list (k from 27 to 30): that
+    build runs a rook kernel and matches the oracle."""
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(41)
+    RBF._lib.tune_reset()
+    pts = rng.random((30_000, 2))
+    RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=30)
+    n = 6000
+    pv = rng.random((n, 2))
+    _, full = cKDTree(pv).query(pv, 30)
+    ks = rng.integers(27, 31, size=n)
+    adjl = [full[i, :ks[i]] for i in range(n)]
+    path = tmp_path / "ragged.txt"
+    # dim n_cols precision bucket kmax m basis hermite uniform kernel
+    path.write_text(f"2 1 0 1 30 6 3 0 0 {K_NSC}\n")  # (basis 3: PHS3)
+    for load in (False, True):
+        if load:
+            RBF._lib.tune_load(path)
+        op = RBF.laplacian(pv, basis=RBF.PHS(3, poly_deg=2), adjl=adjl)
+        nb, kern = RBF._lib.last_build_buckets()
+        assert nb[1] == n and kern[1] != K_NSC, (nb, kern)
+        W = op.weights()
+        sel = rng.choice(n, 200, replace=False)
+        aptr = np.concatenate([[0], np.cumsum([len(adjl[i]) for i in sel])])
+        aidx = np.concatenate([adjl[i] for i in sel])
+        ptr, col, wr, _, _ = O.build(pv, pv[sel], (aptr, aidx), O.basis_struct(3, 2),
+                                     _op(O, 1, [(0, O.abi.OP_LAP, 0, 0, 1.0)]), scaled=True)
+        for j, i in enumerate(sel):
+            got = W.data[W.indptr[i]:W.indptr[i + 1]]
+            ref = wr[0][ptr[j]:ptr[j + 1]]
+            assert np.array_equal(W.indices[W.indptr[i]:W.indptr[i + 1]], col[ptr[j]:ptr[j + 1]])
+            assert np.abs(got - ref).max() <= 1e-9 * np.abs(ref).max(), i
     RBF._lib.tune_reset()
