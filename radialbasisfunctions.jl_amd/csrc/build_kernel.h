@@ -61,10 +61,16 @@ __device__ __forceinline__ double rdlane(double v, int l)
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
+// Every control used here (quad_perm, row_ror, row_mirror, row_half_mirror) reads a
+// lane of the same row of 16, so all source lanes are valid and, with full row and
+// bank masks, every lane is written: the instruction needs no old value to merge into
+// (which would cost a register initialisation per use).
 template <int CTRL>
 __device__ __forceinline__ int dpp_i(int v)
 {
-    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+    static_assert(CTRL <= 0xFF || (CTRL >= 0x121 && CTRL <= 0x12F) || CTRL == 0x140 || CTRL == 0x141,
+                  "a control whose every source lane is inside the row");
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, true);
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp(double v)
