@@ -280,6 +280,18 @@ int rbfx_spmv(const rbfx_spmv_args* args);
 int rbfx_build_weights_async(const rbfx_build_args* args);
 int rbfx_spmv_async(const rbfx_spmv_args* args);
 
+/* Build that allocates nothing (no reference counterpart), for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc): the asynchronous build on args->stream with no earlier blocking
+ * build needed. The null-space kernels' hand-back list lives in a static device area
+ * of the library (one per GPU, 16 MiB, held until the process ends). Limits: plain
+ * FP64 (RBFX_PREC_FP64), at most 4,194,304 stencils (RBFX_EINVAL beyond), and
+ * stencils of varying size (adj_ptr or poly_deg_map) only when k + m <= 32
+ * (RBFX_EUNSUPPORTED). No autotuning: the default or an already tuned kernel runs;
+ * status words are written only to args->status when given. Calls that overlap on
+ * one GPU share the area and race; serialise them (one stream, or events).       */
+int rbfx_build_weights_noalloc(const rbfx_build_args* args);
+
 /* Morton (Z-curve) order of n points (FP64 AoS, dim doubles each): out_order[r]
  * is the index of the r-th point along the curve. The device-side counterpart of
  * reorder_points! (src/utils.jl:45-55), used to order the apply's rows without

@@ -52,6 +52,8 @@ def lib():
                 L.rbfx_spmv.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_build_weights_async.restype = C.c_int
                 L.rbfx_build_weights_async.argtypes = [C.POINTER(abi.BuildArgs)]
+                L.rbfx_build_weights_noalloc.restype = C.c_int
+                L.rbfx_build_weights_noalloc.argtypes = [C.POINTER(abi.BuildArgs)]
                 L.rbfx_spmv_async.restype = C.c_int
                 L.rbfx_spmv_async.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_last_build_counts.restype = C.c_int
@@ -94,6 +96,7 @@ def exported_symbols():
             "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
             "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async",
+            "rbfx_build_weights_noalloc",
             "rbfx_permute_rows", "rbfx_last_build_buckets", "rbfx_tune_bucket", "rbfx_last_build_stats",
             "rbfx_tune_save", "rbfx_tune_load", "rbfx_tune_reset"]
 

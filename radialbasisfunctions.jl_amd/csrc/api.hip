@@ -84,6 +84,13 @@ std::mutex g_mu;
 DeviceState g_dev[64];
 std::map<TuneKey, int> g_tune_saved;  // choices loaded with rbfx_tune_load (any device)
 
+// The static device area of rbfx_build_weights_noalloc: the null-space kernels'
+// hand-back list (up to kNoAllocStencils stencils) and its count. Every GPU the library
+// runs on holds its own copy (16 MiB) until the process ends; calls on one GPU share it.
+constexpr int64_t kNoAllocStencils = int64_t(1) << 22;
+static __device__ __align__(256) int32_t scratch[kNoAllocStencils + 64];
+int32_t* g_scratch_ptr[64];  // the area's address on each device (hipGetSymbolAddress)
+
 DeviceState& dev_state()
 {
     int d = 0;
@@ -260,6 +267,11 @@ thread_local int64_t g_stats[8];       // rbfx_last_build_stats
 constexpr int64_t kTuneMin = 4096;     // stencils in a bucket before its mapping is timed
 constexpr int32_t kTuneSample = 16384;
 
+// How a build runs: blocking (sizes its buffers, reads back statuses), asynchronous on
+// the caller's stream (capturable; needs buffers an earlier blocking build sized), or
+// asynchronous with no allocation at all (buffers in the library's static area).
+enum class Mode { kBlocking, kAsync, kNoAlloc };
+
 // What decides which kernels can serve a launch: the largest stencil of the bucket
 // (kmax points, m monomials), the operator columns, the basis and whether the build has
 // boundary data (Hermite stencils are never solved by the null-space kernels; they hand
@@ -303,7 +315,14 @@ static int rook_for(int bucket, int n_cols)
     return rbfx::wide_fits(bucket == 1 ? 64 : RBFX_MAX_N, n_cols) ? K_WIDE : K_REG64;
 }
 
-static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape& sh, DeviceState& ds, int bucket,
+// Where a null-space launch queues the stencils it hands back (n_eval entries) and
+// their count: the library's grow-only list, or the no-allocation build's static area.
+struct HandBack {
+    int32_t* list;
+    int32_t* count;
+};
+
+static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape& sh, const HandBack& hb, int bucket,
                                    bool fp32, int rows_cap, int64_t n_work, bool async, hipStream_t st)
 {
     rbfx::BuildDev Q = P;
@@ -311,14 +330,14 @@ static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape
     if (is_nsc(kid)) {
         // the null-space launch, then the rook kernel on the stencils it handed back
         const int cap = bucket == 0 ? 32 : (bucket == 1 ? 64 : RBFX_MAX_N);
-        Q.fail_list = ds.nsc_list;
-        Q.fail_count = ds.nsc_count;
-        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.nsc_count);
+        Q.fail_list = hb.list;
+        Q.fail_count = hb.count;
+        hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, hb.count);
         hipError_t e = rbfx::launch_build_nsc(Q, sh.dim, std::min(sh.kmax, cap), sh.m, kid == K_NSC2 ? 1 : 0, n_work, st);
         if (e != hipSuccess) return e;
         int32_t nf = (int32_t)n_work;
         if (!async) {
-            e = read_count(ds.nsc_count, &nf, st);
+            e = read_count(hb.count, &nf, st);
             if (e != hipSuccess) return e;
             g_stats[0] += nf;
         }
@@ -326,10 +345,10 @@ static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape
         rbfx::BuildDev F = Q;
         F.fail_list = P.fail_list;  // (FP32 mode: the caller's list, unused by FP64 kernels)
         F.fail_count = P.fail_count;
-        F.list = ds.nsc_list;
-        F.list_len = ds.nsc_count;
+        F.list = hb.list;
+        F.list_len = hb.count;
         F.packed = nullptr;         // packed offsets follow the launch's own work order
-        return launch_kernel_id(rook_for(bucket, Q.n_cols), F, sh, ds, bucket, false, rows_cap, nf, async, st);
+        return launch_kernel_id(rook_for(bucket, Q.n_cols), F, sh, hb, bucket, false, rows_cap, nf, async, st);
     }
     if (kid == K_WIDE) return rbfx::launch_build_wide(Q, sh.dim, rows_cap, n_work, st);
     const bool f32 = fp32 && (kid == K_REG32 || kid == K_REG64);
@@ -342,10 +361,11 @@ static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape
 // default (a null-space kernel where one serves, else the narrowest rook kernel that
 // fits). Q describes the sample: a work list whose length is sample_len (uneven
 // builds), or the first `sample` stencils in processing order.
-static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const Shape& sh, DeviceState& ds, int bucket,
-                       bool fp32, int rows_cap, int64_t count, int32_t* sample_len, bool async, hipStream_t st,
-                       int* kid)
+static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const Shape& sh, DeviceState& ds,
+                       const HandBack& hb, int bucket, bool fp32, int rows_cap, int64_t count, int32_t* sample_len,
+                       Mode mode, hipStream_t st, int* kid)
 {
+    const bool async = mode != Mode::kBlocking;
     int dflt = bucket == 0 ? K_REG32 : K_WIDE;
     if (bucket == 1 && !rbfx::wide_fits(64, sh.n_cols)) dflt = K_REG64;
     if (kernel_serves(K_NSC, bucket, fp32, sh, false)) dflt = K_NSC;
@@ -358,20 +378,16 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const 
     const TuneKey key{a->dim,         sh.n_cols, a->precision, bucket, sh.kmax, sh.m, sh.basis_kind, sh.hermite ? 1 : 0,
                       sh.uniform ? 1 : 0};
     // a cached or loaded choice is used only where that kernel serves this build
+    // (looked up, never inserted, by a build that allocates nothing)
     auto it = ds.tuned.find(key);
-    if (it != ds.tuned.end() && !kernel_serves(it->second, bucket, fp32, sh, false)) {
-        ds.tuned.erase(it);
-        it = ds.tuned.end();
-    }
-    if (it == ds.tuned.end()) {
-        auto jt = g_tune_saved.find(key);  // a map loaded with rbfx_tune_load
-        if (jt != g_tune_saved.end() && kernel_serves(jt->second, bucket, fp32, sh, false)) {
-            ds.tuned[key] = jt->second;
-            it = ds.tuned.find(key);
-        }
-    }
-    if (it != ds.tuned.end()) {
+    if (it != ds.tuned.end() && kernel_serves(it->second, bucket, fp32, sh, false)) {
         *kid = it->second;
+        return RBFX_OK;
+    }
+    auto jt = g_tune_saved.find(key);  // a map loaded with rbfx_tune_load
+    if (jt != g_tune_saved.end() && kernel_serves(jt->second, bucket, fp32, sh, false)) {
+        if (mode != Mode::kNoAlloc) ds.tuned[key] = jt->second;
+        *kid = jt->second;
         return RBFX_OK;
     }
     if (async || count < kTuneMin) return RBFX_OK;
@@ -397,10 +413,10 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const 
     float best = 0.f, f64_per_stencil = -1.f;
     for (int c = 0; c < nc; ++c) {
         const bool f32c = f32_factor(cands[c]) && Q.fail_count;
-        HIPCHK(launch_kernel_id(cands[c], S, sh, ds, bucket, fp32, rows_cap, sample, false, st));  // warm-up
+        HIPCHK(launch_kernel_id(cands[c], S, sh, hb, bucket, fp32, rows_cap, sample, false, st));  // warm-up
         if (f32c) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, Q.fail_count);
         HIPCHK(hipEventRecord(ds.tev[0], st));
-        HIPCHK(launch_kernel_id(cands[c], S, sh, ds, bucket, fp32, rows_cap, sample, false, st));
+        HIPCHK(launch_kernel_id(cands[c], S, sh, hb, bucket, fp32, rows_cap, sample, false, st));
         HIPCHK(hipEventRecord(ds.tev[1], st));
         HIPCHK(hipEventSynchronize(ds.tev[1]));
         g_stats[1] += 2;
@@ -469,16 +485,21 @@ static int reserve_nsc_list(DeviceState& ds, int64_t n, bool async)
 // size buckets in processing order, and each bucket runs on the kernel chosen for it.
 // (A tuning sample's queued FP32 failures are discarded with the old count.)
 static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Shape& sh, DeviceState& ds, int nmax,
-                     bool fp32, bool async, hipStream_t st)
+                     bool fp32, Mode mode, const HandBack* static_hb, hipStream_t st)
 {
     const int dim = a->dim;
+    const bool async = mode != Mode::kBlocking;
     for (int b = 0; b < 3; ++b) {
         g_bucket_n[b] = 0;
         g_bucket_kernel[b] = -1;
     }
-    {
+    HandBack hb{ds.nsc_list, ds.nsc_count};
+    if (static_hb) {
+        hb = *static_hb;
+    } else {
         const int rc = reserve_nsc_list(ds, a->n_eval, async);
         if (rc != RBFX_OK) return rc;
+        hb = HandBack{ds.nsc_list, ds.nsc_count};
     }
     const bool uneven = (a->adj_ptr || a->poly_deg_map) && nmax > 32;
     if (!uneven) {
@@ -486,16 +507,19 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Sh
         int kid = K_REG32;
         rbfx::BuildDev Q = P;
         int prc = RBFX_OK;
-        Q.packed = pack_offsets(a, P, ds, nmax, a->n_eval, async, st, &prc);
+        // (no packed offsets without allocation: the kernels gather the coordinates)
+        Q.packed = mode == Mode::kNoAlloc ? nullptr : pack_offsets(a, P, ds, nmax, a->n_eval, async, st, &prc);
         if (prc != RBFX_OK) return prc;
-        const int rc = pick_kernel(a, Q, sh, ds, b, fp32, nmax, a->n_eval, nullptr, async, st, &kid);
+        const int rc = pick_kernel(a, Q, sh, ds, hb, b, fp32, nmax, a->n_eval, nullptr, mode, st, &kid);
         if (rc != RBFX_OK) return rc;
         g_bucket_n[b] = a->n_eval;
         g_bucket_kernel[b] = kid;
         if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
-        HIPCHK(launch_kernel_id(kid, Q, sh, ds, b, fp32, nmax, a->n_eval, async, st));
+        HIPCHK(launch_kernel_id(kid, Q, sh, hb, b, fp32, nmax, a->n_eval, async, st));
         return RBFX_OK;
     }
+    if (mode == Mode::kNoAlloc)  // (the entry point refuses these builds before any work)
+        return set_err(RBFX_EUNSUPPORTED, "a build without allocation needs one size class (k + m <= 32 or fixed k)");
     const size_t need = sizeof(int32_t) * 3 * (size_t)a->n_eval;
     if (!async) {
         HIPCHK(grow((void**)&ds.bucket_list, ds.cap_bucket, need));
@@ -519,7 +543,7 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Sh
         Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
         Q.list_len = ds.bucket_count + 3 + b;
         const int rc =
-            pick_kernel(a, Q, sh, ds, b, fp32, rows_cap[b], cnt[b], ds.bucket_count + 3 + b, async, st, &kid[b]);
+            pick_kernel(a, Q, sh, ds, hb, b, fp32, rows_cap[b], cnt[b], ds.bucket_count + 3 + b, mode, st, &kid[b]);
         if (rc != RBFX_OK) return rc;
     }
     if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
@@ -530,13 +554,31 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Sh
         rbfx::BuildDev Q = P;
         Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
         Q.list_len = ds.bucket_count + b;
-        HIPCHK(launch_kernel_id(kid[b], Q, sh, ds, b, fp32, rows_cap[b], cnt[b], async, st));
+        HIPCHK(launch_kernel_id(kid[b], Q, sh, hb, b, fp32, rows_cap[b], cnt[b], async, st));
     }
     return RBFX_OK;
 }
 
-static int build_impl(const rbfx_build_args* a, bool async)
+// Address of the static area on the current device, looked up once per device (host
+// code never passes the array's own name, which is not a device address).
+static int scratch_area(int32_t** out)
 {
+    int d = 0;
+    HIPCHK(hipGetDevice(&d));
+    if (d < 0 || d >= 64) return set_err(RBFX_EDEVICE, "device index %d out of range", d);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_scratch_ptr[d]) {
+        void* ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[d] = static_cast<int32_t*>(ptr);
+    }
+    *out = g_scratch_ptr[d];
+    return RBFX_OK;
+}
+
+static int build_impl(const rbfx_build_args* a, Mode mode)
+{
+    const bool async = mode != Mode::kBlocking;
     if (!a) return set_err(RBFX_EINVAL, "rbfx_build_weights: null args");
     for (int i = 0; i < 16; ++i) g_counts[i] = 0;
     for (int i = 0; i < 4; ++i) g_stage_ms[i] = 0.0;
@@ -573,6 +615,22 @@ static int build_impl(const rbfx_build_args* a, bool async)
     if (a->n_eval == 0) return RBFX_OK;
     if (a->precision < RBFX_PREC_FP64 || a->precision > RBFX_PREC_LEARNED)
         return set_err(RBFX_EUNSUPPORTED, "precision mode %d is not available through rbfx_build_weights", a->precision);
+    HandBack static_hb{nullptr, nullptr};
+    if (mode == Mode::kNoAlloc) {
+        if (a->precision != RBFX_PREC_FP64)
+            return set_err(RBFX_EUNSUPPORTED, "rbfx_build_weights_noalloc: plain FP64 builds only");
+        if (a->n_eval > kNoAllocStencils)
+            return set_err(RBFX_EINVAL, "rbfx_build_weights_noalloc: %lld stencils exceed its maximum of %lld",
+                           (long long)a->n_eval, (long long)kNoAllocStencils);
+        if ((a->adj_ptr || a->poly_deg_map) && a->k + M > 32)
+            return set_err(RBFX_EUNSUPPORTED,
+                           "rbfx_build_weights_noalloc: stencils of varying size need k + m <= 32 (got up to %d)",
+                           a->k + M);
+        int32_t* area = nullptr;
+        const int rc = scratch_area(&area);
+        if (rc != RBFX_OK) return rc;
+        static_hb = HandBack{area, area + kNoAllocStencils};
+    }
     const rbfx_mlp_model* mdl = a->model;
     if (a->precision == RBFX_PREC_LEARNED) {
         // the surrogate covers the configuration it was trained on
@@ -632,7 +690,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
             return set_err(RBFX_EINVAL, "asynchronous build: run one blocking build with these sizes first");
     }
     int32_t* status = a->status;
-    if (!status) {
+    if (!status && mode != Mode::kNoAlloc) {  // (without allocation: no status words unless passed)
         if (!async) HIPCHK(grow((void**)&ds.status, ds.cap_status, need));
         status = ds.status;
     }
@@ -699,7 +757,7 @@ static int build_impl(const rbfx_build_args* a, bool async)
     };
     HIPCHK(mark(0));
     if (a->precision == RBFX_PREC_FP64) {
-        const int rc = solve_all(a, P, sh, ds, nmax, false, async, st);
+        const int rc = solve_all(a, P, sh, ds, nmax, false, mode, mode == Mode::kNoAlloc ? &static_hb : nullptr, st);
         if (rc != RBFX_OK) return rc;
         HIPCHK(mark(1));
         HIPCHK(mark(2));
@@ -768,8 +826,9 @@ static int build_impl(const rbfx_build_args* a, bool async)
                 const int rr = reserve_nsc_list(ds, a->n_eval, async);
                 if (rr != RBFX_OK) return rr;
             }
-            const int rc = pick_kernel(a, Q, sh, ds, b, true, nmax, n_solve,
-                                       ds.bucket_count ? ds.bucket_count + 3 : nullptr, async, st, &kid);
+            const HandBack hb{ds.nsc_list, ds.nsc_count};
+            const int rc = pick_kernel(a, Q, sh, ds, hb, b, true, nmax, n_solve,
+                                       ds.bucket_count ? ds.bucket_count + 3 : nullptr, mode, st, &kid);
             if (rc != RBFX_OK) return rc;
             hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);  // drop tuning failures
             for (int i = 0; i < 3; ++i) {
@@ -778,9 +837,9 @@ static int build_impl(const rbfx_build_args* a, bool async)
             }
             g_bucket_n[b] = async ? -1 : n_solve;
             g_bucket_kernel[b] = kid;
-            if (n_solve > 0) HIPCHK(launch_kernel_id(kid, Q, sh, ds, b, true, nmax, n_solve, async, st));
+            if (n_solve > 0) HIPCHK(launch_kernel_id(kid, Q, sh, hb, b, true, nmax, n_solve, async, st));
         } else {
-            const int rc = solve_all(a, P, sh, ds, nmax, true, async, st);
+            const int rc = solve_all(a, P, sh, ds, nmax, true, mode, nullptr, st);
             if (rc != RBFX_OK) return rc;
         }
         // FP32_REFINED: the FP32 solve is the main kernel (stage 1 stays empty)
@@ -858,6 +917,9 @@ int rbfx_init(int32_t device)
     HIPCHK(hipGetDeviceProperties(&prop, device));
     if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return set_err(RBFX_EDEVICE, "rbfx kernels are built for gfx950; device %d is %s", device, prop.gcnArchName);
+    int32_t* area = nullptr;  // the no-allocation build's static area on this device
+    const int rc = scratch_area(&area);
+    if (rc != RBFX_OK) return rc;
     g_err.clear();
     return RBFX_OK;
 }
@@ -991,9 +1053,11 @@ int rbfx_knn(const double* data, int64_t n_data, const double* query, int64_t n_
     return RBFX_OK;
 }
 
-int rbfx_build_weights(const rbfx_build_args* a) { return build_impl(a, false); }
+int rbfx_build_weights(const rbfx_build_args* a) { return build_impl(a, Mode::kBlocking); }
 
-int rbfx_build_weights_async(const rbfx_build_args* a) { return build_impl(a, true); }
+int rbfx_build_weights_async(const rbfx_build_args* a) { return build_impl(a, Mode::kAsync); }
+
+int rbfx_build_weights_noalloc(const rbfx_build_args* a) { return build_impl(a, Mode::kNoAlloc); }
 
 int64_t rbfx_mlp_param_count(void) { return rbfx::mlp_n_params(); }
 
