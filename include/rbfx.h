@@ -332,8 +332,8 @@ int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3]);
  * an FP32 + refinement build; 5 null-space Cholesky with one FP64 refinement step
  * (interior stencils of bases definite on the polynomial null space: PHS1 with degree
  * >= 0, PHS3 >= 1, PHS5 >= 2, PHS7 >= 3, IMQ, Gaussian; the stencils whose pivots or
- * refinement fail are re-solved by kernel 0 or 2); 6 the same with two rows per lane
- * (k <= 20, at most 6 monomials). A pinned kernel that cannot serve a build is
+ * refinement fail are re-solved by kernel 0 or 2); 6 the same with 8 lanes of three
+ * rows per stencil (k <= 20, at most 6 monomials). A pinned kernel that cannot serve a build is
  * ignored by that build.                                                        */
 int rbfx_tune_bucket(int32_t bucket, int32_t kernel);
 

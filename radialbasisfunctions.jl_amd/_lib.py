@@ -174,8 +174,8 @@ def last_build_buckets():
 def tune_bucket(bucket, kernel):
     """Pin the kernel of a size bucket (-1: automatic). Kernel ids as rbfx_tune_bucket
     documents them: 0 rook registers x2, 1 rook registers (pinned only), 2 rook LDS,
-    3/4 those in FP64 within FP32 mode, 5 null-space Cholesky, 6 the same, two rows
-    per lane."""
+    3/4 those in FP64 within FP32 mode, 5 null-space Cholesky, 6 the same, 8 lanes of
+    three rows per stencil."""
     _check(lib().rbfx_tune_bucket(int(bucket), int(kernel)))
 
 

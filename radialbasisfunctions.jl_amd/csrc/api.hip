@@ -249,7 +249,8 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
 //   K_NSC, K_NSC2  null-space Cholesky (no pivot search) with one FP64 refinement
 //             step, for interior stencils of bases definite on the polynomial null
 //             space; the stencils it hands back are re-solved by K_REG32 / K_WIDE.
-//             K_NSC2 maps two rows per lane (four stencils per wavefront, k <= 20).
+//             K_NSC2 maps 8 lanes of three rows to a stencil (eight stencils per
+//             wavefront, k <= 20, m <= 6).
 enum {
     K_REG32 = 0,
     K_REG64 = 1,

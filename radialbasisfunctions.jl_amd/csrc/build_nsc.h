@@ -27,7 +27,8 @@ __host__ __device__ inline int nsc_sign(int basis_kind, int poly_deg)
 // Whether a null-space kernel of the given variant is built for stencils of at most
 // kmax points, m monomials (the largest over the launch; m < kmax) and n_cols operator
 // columns in dim dimensions. Variant 0 (32 lanes per stencil) is built for one operator
-// column, up to 32 points and 10 monomials; no other variant is built.
+// column, up to 32 points and 10 monomials; variant 1 (8 lanes of three rows per
+// stencil) for one column, up to 20 points and 6 monomials.
 bool nsc_fits(int dim, int kmax, int m, int n_cols, int variant);
 
 // FP64 null-space build of the work (P.list / P.order / all stencils) of a uniform build

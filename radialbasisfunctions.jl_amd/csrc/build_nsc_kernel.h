@@ -119,7 +119,6 @@ struct NscLds {
     static constexpr int NR = GS * SL;        // row-index space of the group
     double xi[NR][DIM];                       // scaled offsets of the stencil points
     double vec[2][NR];                        // broadcast vectors of a step
-    double V[MMAX][NR];                       // Householder reflectors (zero above row c)
     double rhs[RC][NR];                       // right-hand side of the point rows
                                               // (then the residual of the refinement)
     double wsol[RC][NR];                      // solution w, per operator column
@@ -353,7 +352,14 @@ k_nsc(const BuildDev P)
     }
 
     // ---- Householder QR of P, applied to Phi from both sides: Phi <- H Phi H
-    // (H = I - tau v v^T, v zero above row c); the reflectors go to LDS
+    // (H = I - tau v v^T, v zero above row c). Each lane keeps the reflectors' entries
+    // of its own rows (vr, in the registers of the columns of P they replace): the
+    // solves apply Q and Q^T with them and a sum over the group.
+    double vr[SL][MMAX];
+#pragma unroll
+    for (int s = 0; s < SL; ++s)
+#pragma unroll
+        for (int a = 0; a < MMAX; ++a) vr[s][a] = 0.0;
 #pragma unroll
     for (int c = 0; c < MMAX; ++c) {
         if (c < m_wave) {
@@ -378,7 +384,7 @@ k_nsc(const BuildDev P)
                 const int row = li + s * GS;
                 v[s] = !act ? 0.0 : (row > c ? p[s][c] : (row == c ? xc - alpha : 0.0));
                 S.vec[0][row] = v[s];
-                S.V[c][row] = v[s];
+                vr[s][c] = v[s];
             }
             if (li == c % GS) {
                 S.R[c][c] = act ? alpha : 1.0;
@@ -504,7 +510,7 @@ k_nsc(const BuildDev P)
                 double vc[SL], loc = 0.0;
 #pragma unroll
                 for (int s = 0; s < SL; ++s) {
-                    vc[s] = S.V[c][li + s * GS];
+                    vc[s] = vr[s][c];
                     loc = fma(vc[s], bt[s], loc);
                 }
                 const double sp = S.tau[c] * gsum<GS>(loc);
@@ -599,7 +605,7 @@ k_nsc(const BuildDev P)
                 double vc[SL], loc = 0.0;
 #pragma unroll
                 for (int s = 0; s < SL; ++s) {
-                    vc[s] = S.V[c][li + s * GS];
+                    vc[s] = vr[s][c];
                     loc = fma(vc[s], w[s], loc);
                 }
                 const double sp = S.tau[c] * gsum<GS>(loc);

@@ -6,6 +6,7 @@ against the extended-precision oracle at max(1e-10, kappa_2(A) u), the bound the
 kernels meet (test_gpu_build.check_against_truth)."""
 import contextlib
 import pathlib
+from math import comb
 
 import numpy as np
 import pytest
@@ -14,13 +15,13 @@ from test_gpu_build import _c1_points, _op, check_against_truth
 
 pytestmark = pytest.mark.gpu
 ROOT = pathlib.Path(__file__).resolve().parent.parent
-K_REG32, K_WIDE, K_NSC = 0, 2, 5
+K_REG32, K_WIDE, K_NSC, K_NSC2 = 0, 2, 5, 6
 
 
 @contextlib.contextmanager
 def pinned(RBF, kernel, buckets=None):
     if buckets is None:
-        buckets = {K_REG32: (0,), K_WIDE: (1, 2), K_NSC: (0, 1)}[kernel]
+        buckets = {K_REG32: (0,), K_WIDE: (1, 2), K_NSC: (0, 1), K_NSC2: (0,)}[kernel]
     for b in buckets:
         RBF._lib.tune_bucket(b, kernel)
     try:
@@ -43,8 +44,17 @@ CASES = [(3, 0, 2, 2, 20), (3, 0, 2, 1, 12), (1, 0, 2, 0, 12), (1, 0, 2, 2, 20),
          (10, 1.5, 2, 2, 20), (10, 1.5, 3, 1, 16), (11, 1.5, 2, 2, 20), (11, 1.5, 2, -1, 12)]
 
 
-@pytest.mark.parametrize("kind,epsh,dim,deg,k", CASES)
-def test_nsc_parity_against_truth(RBF, O, gpu, kind, epsh, dim, deg, k):
+# (kind, eps h, dim, degree, k, kernel): every case on the 32-lane kernel, and those
+# with k <= 20 and at most 6 monomials on the 8-lane kernel too
+def _n_mono(dim, deg):
+    return comb(dim + deg, dim) if deg >= 0 else 0
+
+
+NSC_CASES = [c + (K_NSC,) for c in CASES] + [c + (K_NSC2,) for c in CASES if c[4] <= 20 and _n_mono(c[2], c[3]) <= 6]
+
+
+@pytest.mark.parametrize("kind,epsh,dim,deg,k,kernel", NSC_CASES)
+def test_nsc_parity_against_truth(RBF, O, gpu, kind, epsh, dim, deg, k, kernel):
     rng = np.random.default_rng(1000 + kind * 37 + dim * 11 + deg * 3 + k)
     n = 1500
     pts = rng.random((n, dim))
@@ -53,10 +63,10 @@ def test_nsc_parity_against_truth(RBF, O, gpu, kind, epsh, dim, deg, k):
         vol = {1: 2.0, 2: np.pi, 3: 4.0 * np.pi / 3.0}[dim]
         eps = epsh / (k / (n * vol)) ** (1.0 / dim)
     L = RBF.Laplacian() + 0.5 * RBF.Partial(1, 1)
-    with pinned(RBF, K_NSC):
+    with pinned(RBF, kernel):
         op = RBF.RadialBasisOperator(L, pts, basis=_basis(RBF, kind, eps, deg), k=k)
     nb, kern = RBF._lib.last_build_buckets()
-    assert K_NSC in kern, (nb, kern)
+    assert kernel in kern, (nb, kern)
     handed = RBF._lib.last_build_stats()[0]
     assert handed <= n // 20, handed  # breakdowns are rare for these pairings
     adj = op.adj_idx.cpu().numpy()
@@ -64,11 +74,12 @@ def test_nsc_parity_against_truth(RBF, O, gpu, kind, epsh, dim, deg, k):
     check_against_truth(O, op.vals.cpu().numpy(), pts, pts, adj, O.basis_struct(kind, deg, eps), _op(O, 1, terms), k)
 
 
-def test_nsc_regrid_and_partial(RBF, O, gpu):
+@pytest.mark.parametrize("kernel", [K_NSC, K_NSC2])
+def test_nsc_regrid_and_partial(RBF, O, gpu, kernel):
     rng = np.random.default_rng(5)
     pts = rng.random((2500, 2))
     ev = rng.random((900, 2))
-    with pinned(RBF, K_NSC):
+    with pinned(RBF, kernel):
         reg = RBF.regrid(pts, ev, basis=RBF.PHS(3, poly_deg=2), k=20)
         par = RBF.partial(pts, 2, 2, basis=RBF.PHS(3, poly_deg=2), k=20)
     b = O.basis_struct(3, 2)
@@ -78,14 +89,15 @@ def test_nsc_regrid_and_partial(RBF, O, gpu):
                         _op(O, 1, [(0, O.abi.OP_D2, 1, 0, 1.0)]), 20)
 
 
-def test_nsc_c1_laplacian(RBF, O, gpu):
+@pytest.mark.parametrize("kernel", [K_NSC, K_NSC2])
+def test_nsc_c1_laplacian(RBF, O, gpu, kernel):
     """The C1 configuration (10,000 points, PHS3, degree 2, k = 20) on the null-space
-    kernel: every stencil within the bound, 99.5% within 1e-10 of the reference's own
+    kernels: every stencil within the bound, 99.5% within 1e-10 of the reference's own
     uncentred arithmetic, and the applied Laplacian as accurate as the rook kernel's."""
     pts = _c1_points()
-    with pinned(RBF, K_NSC):
+    with pinned(RBF, kernel):
         op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20)
-    assert RBF._lib.last_build_buckets()[1][0] == K_NSC
+    assert RBF._lib.last_build_buckets()[1][0] == kernel
     with pinned(RBF, K_REG32):
         rook = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20, adjl=op.adj_idx.cpu().numpy())
     adj = op.adj_idx.cpu().numpy()
@@ -103,14 +115,15 @@ def test_nsc_c1_laplacian(RBF, O, gpu):
     assert abs(e_nsc - e_rook) <= 1e-9 * e_rook
 
 
-def test_nsc_hands_back_hermite_stencils(RBF, O, gpu):
+@pytest.mark.parametrize("kernel", [K_NSC, K_NSC2])
+def test_nsc_hands_back_hermite_stencils(RBF, O, gpu, kernel):
     """Hermite stencils go to the rook kernel; the build matches the oracle and the
     manufactured solution u = x^2 + y^2 + 1 (laplacian_end_to_end.jl:45-78)."""
     from test_oracle import _unit_square_441
 
     pts, isb, nrm, bcs = _unit_square_441()
     adjl = O.knn(pts, pts, 15)
-    with pinned(RBF, K_NSC):
+    with pinned(RBF, kernel):
         op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), adjl=adjl,
                            hermite=dict(is_boundary=isb, bc=[RBF.BoundaryCondition(a, b) for a, b in bcs],
                                         normals=nrm))
@@ -131,10 +144,11 @@ def test_nsc_hands_back_hermite_stencils(RBF, O, gpu):
     assert np.abs(W.data - ws[0]).max() <= 1e-10 * np.abs(ws[0]).max()
 
 
-def test_nsc_repeated_builds_bit_identical(RBF, gpu):
+@pytest.mark.parametrize("kernel", [K_NSC, K_NSC2])
+def test_nsc_repeated_builds_bit_identical(RBF, gpu, kernel):
     pts = np.random.default_rng(99).random((20000, 2))
     first = None
-    with pinned(RBF, K_NSC):
+    with pinned(RBF, kernel):
         for rep in range(8):
             op = RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=2), k=20)
             w = op.vals.clone()
@@ -144,12 +158,13 @@ def test_nsc_repeated_builds_bit_identical(RBF, gpu):
                 assert bool((w == first).all()), rep
 
 
-def test_nsc_singular_stencil_falls_back(RBF, gpu):
+@pytest.mark.parametrize("kernel", [K_NSC, K_NSC2])
+def test_nsc_singular_stencil_falls_back(RBF, gpu, kernel):
     """A duplicated point makes a stencil singular: the null-space kernel hands it back
     and the rook kernel reports it as the reference does (SingularException)."""
     pts = np.random.default_rng(0).random((50, 2))
     pts[1] = pts[0]
-    with pinned(RBF, K_NSC):
+    with pinned(RBF, kernel):
         with pytest.raises(RBF.SingularException):
             RBF.laplacian(pts, k=12)
 

@@ -1,7 +1,7 @@
 """The build that allocates nothing (rbfx_build_weights_noalloc), called through the C
 ABI: two calls in a row, the second with a smaller input, each checked against
-rbfx_build_weights on the same stencils (the same kernel pinned: equal to summation
-rounding; the automatic choice: within the build's accuracy bar), plus a ragged
+rbfx_build_weights on the same stencils with the same kernel (pinned, or the one the
+blocking build tuned), equal to summation-order rounding, plus a ragged
 adjacency list of one size class, and the limits the entry point refuses before doing
 any work."""
 import ctypes as C
@@ -87,17 +87,19 @@ def test_noalloc_build_matches_blocking_build(RBF, gpu, pin):
             rng = np.random.default_rng(seed)
             pts = rng.random((n, 2))
             adj = RBF.find_neighbors(pts, k=20, device=gpu)
+            # the blocking build first: unpinned, it tunes this configuration, and the
+            # build without allocation then runs the kernel it picked
+            b, keep2, vals2, _ = _args(RBF, gpu, pts, adj, basis, terms)
+            _blocking(RBF, gpu, b)
+            ref = vals2.cpu().numpy()
             a, keep, vals, status = _args(RBF, gpu, pts, adj, basis, terms)
             assert _noalloc(RBF, gpu, a) == 0, RBF._lib.last_error()
             w = vals.cpu().numpy()
             st = status.cpu().numpy()
             assert np.all(np.isfinite(w)) and np.all(st == A.ST_OK), np.unique(st)
-            b, keep2, vals2, _ = _args(RBF, gpu, pts, adj, basis, terms)
-            _blocking(RBF, gpu, b)
-            ref = vals2.cpu().numpy()
-            # the same kernel pinned: the same arithmetic up to where the offsets come from;
-            # the automatic choice may differ in kernel: both within 1e-10 of each other
-            assert _rows_close(w, ref, 20, 1e-13 if pin else 1e-10), n
+            # one kernel on the same stencils; its offsets gathered in the kernel instead
+            # of packed ahead (the same subtraction): equal to summation-order rounding
+            assert _rows_close(w, ref, 20, 1e-13), n
     finally:
         RBF._lib.tune_bucket(0, -1)
 
@@ -116,12 +118,12 @@ def test_noalloc_ragged_one_size_class(RBF, gpu):
     idx = np.concatenate([full[i, :ks[i]] for i in range(n)])
     basis = A.Basis(3, 2, 0.0)
     terms = [(0, A.OP_D1, 0, 0, 1.0)]
-    a, keep, vals, status = _args(RBF, gpu, pts, idx, basis, terms, adj_ptr=ptr)
-    assert _noalloc(RBF, gpu, a) == 0, RBF._lib.last_error()
-    w = vals.cpu().numpy()
     b, keep2, vals2, _ = _args(RBF, gpu, pts, idx, basis, terms, adj_ptr=ptr)
     _blocking(RBF, gpu, b)
     ref = vals2.cpu().numpy()
+    a, keep, vals, status = _args(RBF, gpu, pts, idx, basis, terms, adj_ptr=ptr)
+    assert _noalloc(RBF, gpu, a) == 0, RBF._lib.last_error()
+    w = vals.cpu().numpy()
     assert np.all(np.isfinite(w))
     for i in rng.choice(n, 2000, replace=False):
         s = slice(ptr[i], ptr[i + 1])
