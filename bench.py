@@ -174,7 +174,8 @@ def _build_ms(R, op, reps=2):
 
 def _status(R, op):
     sc = op.status_counts
-    return {"ok": sc[0], "dirichlet": sc[1], "refined": sc[2], "fallback64": sc[3], "fp64": sc[5]}
+    return {"ok": sc[0], "dirichlet": sc[1], "refined": sc[2], "fallback64": sc[3], "fp64": sc[5],
+            "nsc_handed_back": R._lib.last_build_stats()[0]}
 
 
 def config_c2(R, dev):
@@ -370,6 +371,9 @@ def main():
     ap.add_argument("--model", default=str(MODEL_PATH))
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (C2, C3, C5)")
     ap.add_argument("--only", choices=["c2", "c3", "c5"], help="run one other configuration alone and print it")
+    ap.add_argument("--kernel", type=int, default=-1,
+                    help="pin this solve kernel id in every size bucket it serves instead of autotuning "
+                         "(profiling runs: every counter pass then sees the same launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU; multi-GPU figures are not produced "
@@ -384,6 +388,14 @@ def main():
     R = rbfx_loader.load()
     L = R.learned
     dev = R._lib.device_for("cuda:0")
+    pinned = {}
+    if args.kernel >= 0:
+        for b in range(3):
+            try:
+                R._lib.tune_bucket(b, args.kernel)
+                pinned[b] = args.kernel
+            except Exception:
+                pass
     if args.only:
         fn = {"c2": config_c2, "c3": config_c3, "c5": lambda R, d: config_c5(R, d, torch.cuda.current_stream(d))}
         print(json.dumps({args.only: fn[args.only](R, dev)}))
@@ -429,7 +441,7 @@ def main():
     # automatic choice solves this size class in FP64 when that measures faster)
     for method in ("fp64", "fp32_refined", "learned", "fp32_kernel"):
         prec = "fp32_refined" if method == "fp32_kernel" else method
-        R._lib.tune_bucket(0, 0 if method == "fp32_kernel" else -1)
+        R._lib.tune_bucket(0, 0 if method == "fp32_kernel" else pinned.get(0, -1))
         op = R.RadialBasisOperator(R.Laplacian(), d_pts, basis=R.PHS(3, poly_deg=deg), k=k, adjl=adj,
                                    precision=prec, model=model if method == "learned" else None)
         for _ in range(args.warmup):
@@ -460,7 +472,7 @@ def main():
         if method == "learned":
             learned_kernel = kern
         ops[method] = op
-    R._lib.tune_bucket(0, -1)
+    R._lib.tune_bucket(0, pinned.get(0, -1))
 
     # apply: y = W f (Laplacian, 1 field), rows walked in spatial order (apply plan)
     op = ops["fp64"]
