@@ -371,6 +371,9 @@ def main():
     ap.add_argument("--model", default=str(MODEL_PATH))
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (C2, C3, C5)")
     ap.add_argument("--only", choices=["c2", "c3", "c5"], help="run one other configuration alone and print it")
+    ap.add_argument("--learned-sweep", default="",
+                    help="comma-separated acceptance tolerances: the learned build at each (accepted "
+                         "fraction, error ratio, time), the ablation behind the learned result")
     ap.add_argument("--kernel", type=int, default=-1,
                     help="pin this solve kernel id in every size bucket it serves instead of autotuning "
                          "(profiling runs: every counter pass then sees the same launches)")
@@ -468,11 +471,34 @@ def main():
                                stencils_per_s=args.n / (gpu_ms * 1e-3), nrmse=nrmse,
                                stage_ms=stage_ms[method], solve_kernel=KERNEL_NAMES[kern],
                                status={"ok": sc[0], "refined": sc[2], "fallback64": sc[3], "learned": sc[4],
-                                       "fp64": sc[5]})
+                                       "fp64": sc[5]},
+                               nsc_handed_back=R._lib.last_build_stats()[0])
         if method == "learned":
             learned_kernel = kern
         ops[method] = op
     R._lib.tune_bucket(0, pinned.get(0, -1))
+
+    sweep = None
+    if args.learned_sweep:
+        # the same network at other acceptance tolerances: what accepting more costs in error
+        sweep = []
+        tol0 = model.accept_tol
+        op = ops["learned"]
+        for tol in [float(t) for t in args.learned_sweep.split(",")]:
+            model.accept_tol = tol
+            op.update_weights()
+            torch.cuda.synchronize()
+            ts = []
+            for _ in range(max(1, args.steps)):
+                op.update_weights()
+                ts.append(R._lib.last_build_timing()[3])
+            y = op(d_f).cpu().numpy()
+            nr = float(np.sqrt(np.mean((y - exact) ** 2)) / np.sqrt(np.mean(exact ** 2)))
+            ms = float(np.median(ts))
+            sweep.append({"accept_tol": tol, "accepted_fraction": op.status_counts[4] / args.n,
+                          "error_ratio_vs_plain": nr / results["fp64"]["nrmse"], "ms_per_build": ms,
+                          "speedup_vs_plain_fp64": results["fp64"]["ms_per_build"] / ms})
+        model.accept_tol = tol0
 
     # apply: y = W f (Laplacian, 1 field), rows walked in spatial order (apply plan)
     op = ops["fp64"]
@@ -563,6 +589,7 @@ def main():
         "error_ratio_vs_plain": ratio,
         "equal_error_met": bool(ratio <= 1.05),
         "accepted_fraction": n_acc / args.n,
+        "learned_sweep": sweep,
         "methods": results,
         "model": model_info,
         "knn_ms": knn_s * 1e3,

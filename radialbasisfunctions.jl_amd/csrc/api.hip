@@ -370,6 +370,7 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const 
     int dflt = bucket == 0 ? K_REG32 : K_WIDE;
     if (bucket == 1 && !rbfx::wide_fits(64, sh.n_cols)) dflt = K_REG64;
     if (kernel_serves(K_NSC, bucket, fp32, sh, false)) dflt = K_NSC;
+    if (kernel_serves(K_NSC2, bucket, fp32, sh, false)) dflt = K_NSC2;  // (the faster mapping where it fits)
     *kid = dflt;
     const int forced = g_force_kernel[bucket];
     if (forced >= 0 && kernel_serves(forced, bucket, fp32, sh, true)) {

@@ -18,7 +18,7 @@ bool nsc_fits(int dim, int kmax, int m, int n_cols, int variant)
 {
     if (dim < 1 || dim > 3 || n_cols != 1 || kmax < 1 || m < 0 || m >= kmax) return false;
     if (variant == 0) return kmax <= 32 && m <= 10;
-    if (variant == 1) return kmax <= 20 && m <= 6;
+    if (variant == 1) return kmax <= 20 && m <= (dim == 3 ? 4 : 6);
     return false;
 }
 

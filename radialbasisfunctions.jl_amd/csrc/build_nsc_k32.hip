@@ -1,5 +1,6 @@
 // Instantiates the null-space Cholesky build for stencils of up to 32 points and up to
-// 10 monomials: 32 lanes per stencil, 1 row(s) per lane, 2 waves per SIMD.
+// 10 monomials: 32 lanes per stencil, 1 row(s) per lane, 2 waves per SIMD, one FP64
+// refinement step.
 #include "build_nsc_kernel.h"
 
 namespace rbfx {
@@ -7,7 +8,7 @@ namespace rbfx {
 template <int DIM>
 static hipError_t nsc_rc_k32(const BuildDev& P, int64_t n_work, hipStream_t s)
 {
-    return launch_nsc_one<32, 1, 32, 10, DIM, 1, 2>(P, n_work, s);
+    return launch_nsc_one<32, 1, 32, 10, DIM, 1, 1, 2>(P, n_work, s);
 }
 
 hipError_t launch_nsc_k32(const BuildDev& P, int dim, int64_t n_work, hipStream_t s)

@@ -120,17 +120,18 @@ struct NscLds {
     double xi[NR][DIM];                       // scaled offsets of the stencil points
     double vec[2][NR];                        // broadcast vectors of a step
     double rhs[RC][NR];                       // right-hand side of the point rows
-                                              // (then the residual of the refinement)
+    double res[RC][NR];                       // residual of a refinement step (point rows)
     double wsol[RC][NR];                      // solution w, per operator column
     double wt[RC][NR];                        // the last solve's Q^T w
     double piv[KMAX + 2];                     // pivot row of an LDL^T step / a sent value
     double R[MMAX][MMAX];                     // triangular factor of P
     double tau[MMAX];                         // reflector scales
     double g[RC][MMAX];                       // polynomial right-hand side
+    double gres[RC][MMAX];                    // residual of a refinement step (polynomial rows)
     double w1[MMAX];                          // R^-T g of the column being solved
     double btm[RC][MMAX];                     // the last solve's (Q^T rhs)[:M]
     double lam[RC][MMAX];                     // Lagrange multipliers
-    double dlam[RC][MMAX];                    // their refinement correction
+    double dlam[RC][MMAX];                    // their correction in a refinement step
 };
 
 // ------------------------------------------------------------------ kernel
@@ -497,9 +498,11 @@ k_nsc(const BuildDev P)
     // (a group that failed fails as a whole: every lane saw the same values)
 
     // ---- solve with the factors, one operator column at a time: rhs on the point rows
-    // and the polynomial RHS in S.g[cc]; leaves w in S.wsol[cc] (add: += to it), Q^T w
-    // in S.wt[cc] and (Q^T rhs)[:M] in S.btm[cc]; returns this lane's largest |w|
-    auto solve_col = [&](const int cc, const double (&rhs)[SL], const bool add) __attribute__((always_inline)) {
+    // and the polynomial RHS gp (S.g[cc], or S.gres[cc] in a refinement step); leaves w
+    // in S.wsol[cc] (add: += to it), Q^T w in S.wt[cc] and (Q^T rhs)[:M] in S.btm[cc];
+    // returns this lane's largest |w|
+    auto solve_col = [&](const int cc, const double (&rhs)[SL], const bool add, const double* gp)
+                         __attribute__((always_inline)) {
         double bt[SL];
 #pragma unroll
         for (int s = 0; s < SL; ++s) bt[s] = rhs[s];
@@ -527,7 +530,7 @@ k_nsc(const BuildDev P)
             const int as = a / GS, al = a % GS;
             if (a < m_wave) {
                 if (li == al) {
-                    S.w1[a] = a < M ? (S.g[cc][a] - acc[as]) * nsc_rcp(S.R[a][a]) : 0.0;
+                    S.w1[a] = a < M ? (gp[a] - acc[as]) * nsc_rcp(S.R[a][a]) : 0.0;
                     S.btm[cc][a] = bt[as];
                 }
                 __syncthreads();
@@ -667,29 +670,40 @@ k_nsc(const BuildDev P)
         double rhs[SL];
 #pragma unroll
         for (int s = 0; s < SL; ++s) rhs[s] = S.rhs[cc][li + s * GS];
-        solve_col(cc, rhs, false);
+        solve_col(cc, rhs, false, S.g[cc]);
         if (NREF > 0 || P.out_lambda) multipliers(cc, S.lam);
     }
 
     if constexpr (NREF > 0) {
-        // residual of the assembled (untransformed) system at (w, lambda), FP64
-        double r[SL][RC];
+        // FP64 refinement on the assembled (untransformed) system: the residual at
+        // (w, lambda), a correction through the same factors, w += dw, lambda += dlambda.
+        // Up to NREF steps, until the correction of every group of the wavefront is
+        // below 1e-7 of its solution; a group whose correction does not settle goes to the
+        // rook-pivoted kernel. Each step contracts the error by about the relative
+        // accuracy of the factors. Where that is poor, for nearly flat kernels (an IMQ
+        // with eps h of a few hundredths: eigenvalues of the null-space block from 1e-5
+        // down to 1e-14 against |Phi| ~ k), it stays near 1 and more steps do not help,
+        // so every built kernel takes one step (NREF = 1).
+        int settled = 0;
+#pragma unroll 1
+        for (int it = 0; it < NREF; ++it) {
+            double r[SL][RC];
 #pragma unroll
-        for (int s = 0; s < SL; ++s) {
+            for (int s = 0; s < SL; ++s) {
 #pragma unroll
-            for (int c = 0; c < RC; ++c) r[s][c] = S.rhs[c][li + s * GS];
+                for (int c = 0; c < RC; ++c) r[s][c] = S.rhs[c][li + s * GS];
 #pragma unroll
-            for (int a = 0; a < MMAX; ++a) {
-                if (a < m_wave) {
-                    const double pa = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
+                for (int a = 0; a < MMAX; ++a) {
+                    if (a < m_wave) {
+                        const double pa = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
 #pragma unroll
-                    for (int c = 0; c < RC; ++c) r[s][c] = fma(-pa, S.lam[c][a], r[s][c]);
+                        for (int c = 0; c < RC; ++c) r[s][c] = fma(-pa, S.lam[c][a], r[s][c]);
+                    }
+                    __builtin_amdgcn_sched_barrier(0);
                 }
-                __builtin_amdgcn_sched_barrier(0);
             }
-        }
-// (the compiler barrier at the top keeps the solution's LDS loads inside each basis
-// case: loads common to all cases would otherwise be hoisted above the switch and held)
+    // (the compiler barrier at the top keeps the solution's LDS loads inside each basis
+    // case: loads common to all cases would otherwise be hoisted above the switch and held)
 #define RBFX_NSC_RESID(BK)                                                               \
     asm volatile("" ::: "memory");                                                       \
     _Pragma("unroll") for (int j = 0; j < KMAX; ++j)                                     \
@@ -708,60 +722,71 @@ k_nsc(const BuildDev P)
             _Pragma("unroll") for (int c = 0; c < RC; ++c) r[s][c] = fma(-v_, S.wsol[c][j], r[s][c]); \
         }                                                                                \
         __builtin_amdgcn_sched_barrier(0);                                               \
-    }
-        switch (kind) {
-        case RBFX_PHS1: RBFX_NSC_RESID(RBFX_PHS1) break;
-        case RBFX_PHS3: RBFX_NSC_RESID(RBFX_PHS3) break;
-        case RBFX_PHS5: RBFX_NSC_RESID(RBFX_PHS5) break;
-        case RBFX_PHS7: RBFX_NSC_RESID(RBFX_PHS7) break;
-        case RBFX_IMQ: RBFX_NSC_RESID(RBFX_IMQ) break;
-        default: RBFX_NSC_RESID(RBFX_GAUSSIAN) break;
         }
+            switch (kind) {
+            case RBFX_PHS1: RBFX_NSC_RESID(RBFX_PHS1) break;
+            case RBFX_PHS3: RBFX_NSC_RESID(RBFX_PHS3) break;
+            case RBFX_PHS5: RBFX_NSC_RESID(RBFX_PHS5) break;
+            case RBFX_PHS7: RBFX_NSC_RESID(RBFX_PHS7) break;
+            case RBFX_IMQ: RBFX_NSC_RESID(RBFX_IMQ) break;
+            default: RBFX_NSC_RESID(RBFX_GAUSSIAN) break;
+            }
 #undef RBFX_NSC_RESID
-        // polynomial rows: g - P^T w, into S.g
+            // polynomial rows: g - P^T w, into S.gres
 #pragma unroll
-        for (int a = 0; a < MMAX; ++a) {
-            if (a < m_wave) {
-                double pa[SL];
+            for (int a = 0; a < MMAX; ++a) {
+                if (a < m_wave) {
+                    double pa[SL];
 #pragma unroll
-                for (int s = 0; s < SL; ++s)
-                    pa[s] = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
+                    for (int s = 0; s < SL; ++s)
+                        pa[s] = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
 #pragma unroll
-                for (int c = 0; c < RC; ++c) {
-                    if (c < n_cols) {
-                        double loc = 0.0;
+                    for (int c = 0; c < RC; ++c) {
+                        if (c < n_cols) {
+                            double loc = 0.0;
 #pragma unroll
-                        for (int s = 0; s < SL; ++s) loc = fma(pa[s], S.wsol[c][li + s * GS], loc);
-                        const double ptw = gsum<GS>(loc);
-                        if (li == a % GS) S.g[c][a] = a < M ? S.g[c][a] - ptw : 0.0;
+                            for (int s = 0; s < SL; ++s) loc = fma(pa[s], S.wsol[c][li + s * GS], loc);
+                            const double ptw = gsum<GS>(loc);
+                            if (li == a % GS) S.gres[c][a] = a < M ? S.g[c][a] - ptw : 0.0;
+                        }
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);
             }
-            __builtin_amdgcn_sched_barrier(0);
-        }
-        __syncthreads();
-        // the correction: it must be small against the solution, else the factors are too
-        // inaccurate for this system and the rook-pivoted kernel solves it
-        __syncthreads();
 #pragma unroll
-        for (int s = 0; s < SL; ++s)
+            for (int s = 0; s < SL; ++s)
 #pragma unroll
-            for (int c = 0; c < RC; ++c) S.rhs[c][li + s * GS] = r[s][c];
-        __syncthreads();
-        double dmax = 0.0, wmax = 0.0;
+                for (int c = 0; c < RC; ++c) S.res[c][li + s * GS] = r[s][c];
+            __syncthreads();
+            double dmax = 0.0, wmax = 0.0;
 #pragma unroll 1
-        for (int cc = 0; cc < n_cols; ++cc) {
-            double rhs[SL];
+            for (int cc = 0; cc < n_cols; ++cc) {
+                double rhs[SL];
 #pragma unroll
-            for (int s = 0; s < SL; ++s) rhs[s] = S.rhs[cc][li + s * GS];
-            dmax = fmax(dmax, solve_col(cc, rhs, true));
-            if (P.out_lambda) multipliers(cc, S.dlam);
+                for (int s = 0; s < SL; ++s) rhs[s] = S.res[cc][li + s * GS];
+                dmax = fmax(dmax, solve_col(cc, rhs, true, S.gres[cc]));
+                // (lambda enters the next step's residual; after a single step it is only
+                // an output)
+                if (NREF > 1 || P.out_lambda) {
+                    multipliers(cc, S.dlam);
 #pragma unroll
-            for (int s = 0; s < SL; ++s) wmax = fmax(wmax, fabs(S.wsol[cc][li + s * GS]));
+                    for (int s = 0; s < SL; ++s) {
+                        const int a = li + s * GS;
+                        if (a < MMAX) S.lam[cc][a] += S.dlam[cc][a];
+                    }
+                    __syncthreads();
+                }
+#pragma unroll
+                for (int s = 0; s < SL; ++s) wmax = fmax(wmax, fabs(S.wsol[cc][li + s * GS]));
+            }
+            dmax = gmax<GS>(dmax);
+            wmax = gmax<GS>(wmax);
+            settled = dmax <= 1e-7 * wmax;
+            // (groups without work count as settled; the wavefront leaves together)
+            if constexpr (NREF > 1)
+                if (__ballot(alive && !settled) == 0ull) break;
         }
-        dmax = gmax<GS>(dmax);
-        wmax = gmax<GS>(wmax);
-        if (!(dmax <= 1e-7 * wmax)) fail = 1;
+        if (!settled) fail = 1;
     }
 
     // ---- outputs
@@ -779,8 +804,7 @@ k_nsc(const BuildDev P)
 #pragma unroll
                 for (int c = 0; c < RC; ++c)
                     if (c < n_cols)
-                        P.out_lambda[((int64_t)c * P.n_eval + e) * P.n_mono + row] =
-                            NREF > 0 ? S.lam[c][row] + S.dlam[c][row] : S.lam[c][row];
+                        P.out_lambda[((int64_t)c * P.n_eval + e) * P.n_mono + row] = S.lam[c][row];
             }
         }
         if (li == 0 && P.status) P.status[e] = P.ok_status;
@@ -792,14 +816,16 @@ k_nsc(const BuildDev P)
     }
 }
 
-template <int GS, int SL, int KMAX, int MMAX, int DIM, int RC, int W>
+// NREF: refinement steps at most (a loop of steps keeps every factor register live
+// around it: the 8-lane mapping then spills, the 32-lane ones lose a wave per SIMD)
+template <int GS, int SL, int KMAX, int MMAX, int DIM, int RC, int NREF, int W>
 inline hipError_t launch_nsc_one(const BuildDev& P, int64_t n_work, hipStream_t s)
 {
     if (n_work <= 0) return hipSuccess;
     constexpr int G = 64 / GS;
     BuildDev Q = P;
     Q.n_blocks = (n_work + G - 1) / G;
-    hipLaunchKernelGGL((k_nsc<GS, SL, KMAX, MMAX, DIM, RC, 1, W>), dim3((unsigned)pad_to_xcds(Q.n_blocks)), dim3(64),
+    hipLaunchKernelGGL((k_nsc<GS, SL, KMAX, MMAX, DIM, RC, NREF, W>), dim3((unsigned)pad_to_xcds(Q.n_blocks)), dim3(64),
                        0, s, Q);
     return hipGetLastError();
 }
