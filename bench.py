@@ -178,6 +178,75 @@ def _status(R, op):
             "nsc_handed_back": R._lib.last_build_stats()[0]}
 
 
+def _mono_exps(dim, deg):
+    """Exponents of the monomials of total degree <= deg (any order spans the space)."""
+    import itertools
+
+    return [e for t in range(deg + 1) for e in itertools.product(range(t + 1), repeat=dim) if sum(e) == t]
+
+
+def _lp_at_centre(e, terms, col):
+    """The operator column `col` applied to the centred monomial x^e, at the centre."""
+    from radialbasisfunctions_jl_amd import _abi as A
+
+    dim = len(e)
+    unit = [tuple(int(i == d) for i in range(dim)) for d in range(dim)]
+    v = 0.0
+    for c, kind, d1, d2, coeff in terms:
+        if c != col:
+            continue
+        if kind == A.OP_VALUE:
+            v += coeff * (sum(e) == 0)
+        elif kind == A.OP_D1:
+            v += coeff * (e == unit[d1])
+        elif kind == A.OP_D2 or (kind == A.OP_MIXED and d1 == d2):
+            v += coeff * 2.0 * (e == tuple(2 * u for u in unit[d1]))
+        elif kind == A.OP_MIXED:
+            v += coeff * (e == tuple(a + b for a, b in zip(unit[d1], unit[d2])))
+        elif kind == A.OP_LAP:
+            v += coeff * 2.0 * any(e == tuple(2 * u for u in unit[d]) for d in range(dim))
+    return v
+
+
+def poly_residual(op, data, ev, deg, keep=None, chunk=1 << 20):
+    """Polynomial reproduction of built weights, sum_j w_j p(x_j - x_e) - L p(x_e) for every
+    monomial p of degree <= deg and every operator column, in the unscaled frame (the
+    reference's constraint rows P^T w = L P, assembly.jl:95-185; test/utils.jl:175-180):
+    max |residual| and max |residual| / (sum_j |w_j p(x_j - x_e)| + |L p|), over the
+    stencils selected by `keep` (a boolean mask; Hermite stencils have modified rows)."""
+    import torch
+
+    n_cols, terms = op.L.lower(data.shape[1])
+    E = _mono_exps(data.shape[1], deg)
+    lp = torch.tensor([[_lp_at_centre(e, terms, c) for e in E] for c in range(n_cols)], dtype=torch.float64,
+                      device=data.device)
+    n = ev.shape[0]
+    if op.row_ptr is None:
+        ptr = torch.arange(0, (n + 1) * op.k, op.k, device=data.device, dtype=torch.int64)
+    else:
+        ptr = op.row_ptr
+    col = op.col.long()
+    worst_abs = worst_rel = 0.0
+    for r0 in range(0, n, chunk):
+        r1 = min(n, r0 + chunk)
+        a, b = int(ptr[r0]), int(ptr[r1])
+        rows = torch.repeat_interleave(torch.arange(r0, r1, device=data.device), ptr[r0 + 1:r1 + 1] - ptr[r0:r1])
+        D = data[col[a:b]] - ev[rows]
+        M = torch.stack([torch.prod(D ** torch.tensor(e, dtype=torch.float64, device=data.device), 1) for e in E], 1)
+        sel = torch.ones(r1 - r0, dtype=torch.bool, device=data.device) if keep is None else keep[r0:r1]
+        for c in range(n_cols):
+            wm = op.vals[c, a:b, None] * M
+            ptw = torch.zeros((r1 - r0, len(E)), dtype=torch.float64, device=data.device)
+            mag = torch.zeros_like(ptw)
+            ptw.index_add_(0, rows - r0, wm)
+            mag.index_add_(0, rows - r0, wm.abs())
+            res = (ptw - lp[c])[sel]
+            if res.numel():
+                worst_abs = max(worst_abs, float(res.abs().max()))
+                worst_rel = max(worst_rel, float((res.abs() / (mag[sel] + lp[c].abs())).max()))
+    return {"max_abs": worst_abs, "max_rel": worst_rel, "monomials": len(E), "stencils": int(n if keep is None else keep.sum())}
+
+
 def config_c2(R, dev):
     """3-D gradient + Laplacian: 126^3 jittered grid (2,000,376 points), PHS(5),
     degree 3, k = 40, one fused build with 4 RHS columns (n = 60, r = 4)."""
@@ -210,6 +279,7 @@ def config_c2(R, dev):
             inner = ((d > 0.1) & (d < 0.9)).all(1)
             e = ((G - ex)[inner] ** 2).mean(0).sqrt() / (ex[inner] ** 2).mean(0).sqrt()
             out["nrmse_dx_dy_dz_lap"] = [float(v) for v in e.cpu()]
+            out["poly_residual"] = poly_residual(op, d, d, 3)
             del op, G
     return out
 
@@ -286,7 +356,71 @@ def config_c3(R, dev):
             inner = (x ** 2 + y ** 2) < 0.36
             v = op(f)
             out["nrmse_inner"] = float(((v - ex)[inner] ** 2).mean().sqrt() / (ex[inner] ** 2).mean().sqrt())
+        if prec == "fp64":
+            # stencils without boundary points (degree 2): those with one carry Neumann rows
+            # in place of plain ones, a different constraint
+            out["poly_residual_interior"] = poly_residual(op, d, d, 2, keep=~touch)
         del op
+    return out
+
+
+def config_c3_literal(R, dev):
+    """C3 exactly as specified, with nothing added: the 1,000,000-point Gaussian mixture
+    clipped to the disk and 20,000 uniformly spaced boundary nodes with Neumann rows, k
+    from local density in [12, 60] for every point, degree 3 for the stencils that touch
+    the boundary and 2 elsewhere, dx + d2y, FP32 factorisation with FP64 refinement. A
+    boundary node's stencil of a few circle neighbours is rank-deficient for degree >= 2
+    (x^2 + y^2 - 1 vanishes on it): the reference's bunchkaufman! raises
+    SingularException for such a system (assembly.jl:220-222), and so does this build.
+    Reported: the exception, the number of singular stencils and the build time."""
+    import ctypes as C
+
+    import torch
+
+    rng = np.random.default_rng(20260103)
+    cen = rng.uniform(-0.6, 0.6, (8, 2))
+    sig = rng.uniform(0.02, 0.2, 8)
+    parts, got = [], 0
+    while got < 1_000_000:
+        j = rng.integers(8, size=1_500_000)
+        q = cen[j] + sig[j, None] * rng.standard_normal((len(j), 2))
+        q = q[(q ** 2).sum(1) < 0.995]
+        parts.append(q)
+        got += len(q)
+    mix = np.concatenate(parts)[:1_000_000]
+    nb = 20_000
+    th = 2 * np.pi * np.arange(nb) / nb
+    bnd = np.stack([np.cos(th), np.sin(th)], 1)
+    pts = np.concatenate([mix, bnd])
+    n = len(pts)
+    isb = np.zeros(n, dtype=bool)
+    isb[-nb:] = True
+    d = torch.as_tensor(pts, device=dev)
+    d12 = (d - d[R._lib.knn(d, d, 13)[:, -1].long()]).norm(dim=1)
+    kk = torch.clamp((24 * (d12.median() / d12) ** 2).long(), 12, 60)
+    ptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    ptr[1:] = torch.cumsum(kk, 0)
+    idx = R._lib.knn(d, d, 60, q_ptr=ptr)
+    isb_d = torch.as_tensor(isb, device=dev)
+    hits = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    hits[1:] = torch.cumsum(isb_d[idx.long()].long(), 0)[ptr[1:] - 1]
+    touch = (hits[1:] - hits[:-1]) > 0
+    deg = torch.where(touch, 3, 2).to(torch.int8)
+    herm = R.Hermite(isb, [R.Neumann()] * nb, bnd)
+    out = {"workload": "dx+d2y-2d-disk-1.02M-phs3-deg2/3-k12..60-hermite-neumann (as specified)", "n_points": n,
+           "n_boundary": nb, "k_range": [int(kk.min()), int(kk.max())],
+           "k_at_boundary_nodes": [int(kk[isb_d].min()), int(kk[isb_d].max())]}
+    try:
+        R.RadialBasisOperator(R.Partial(1, 1) + R.Partial(2, 2), d, basis=R.PHS(3, poly_deg=3), adjl=(ptr, idx),
+                              hermite=herm, precision="fp32_refined", poly_deg_map=deg)
+        out["raised"] = None
+    except R.SingularException as ex:
+        out["raised"] = "SingularException: " + str(ex)
+    counts = (C.c_int64 * 16)()
+    R._lib.lib().rbfx_last_build_counts(counts)
+    out["status"] = {"ok": counts[0], "refined": counts[2], "fallback64": counts[3], "fp64": counts[5],
+                     "singular": counts[8]}
+    out["build_ms"] = R._lib.last_build_timing()[3]
     return out
 
 
@@ -347,6 +481,7 @@ def config_c5(R, dev, stream):
     out["fp64"] = {"build_ms": _build_ms(R, op, reps=1), "status": _status(R, op),
                    "kernel": KERNEL_NAMES[max(R._lib.last_build_buckets()[1])], "sources": "after reorder_points"}
     out["fp64"]["stencils_per_s"] = n_t / (out["fp64"]["build_ms"] * 1e-3)
+    out["fp64"]["poly_residual"] = poly_residual(op, ds, dt, 1)
     X = fields(ds).contiguous()
     q_ms = time_apply(R, op, X, stream, reps=10)
     out["apply"]["reordered_sources"] = {"ms": q_ms, "GB_s": by / (q_ms * 1e-3) / 1e9,
@@ -370,7 +505,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--model", default=str(MODEL_PATH))
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (C2, C3, C5)")
-    ap.add_argument("--only", choices=["c2", "c3", "c5"], help="run one other configuration alone and print it")
+    ap.add_argument("--only", choices=["c2", "c3", "c3_literal", "c5"],
+                    help="run one other configuration alone and print it")
     ap.add_argument("--learned-sweep", default="",
                     help="comma-separated acceptance tolerances: the learned build at each (accepted "
                          "fraction, error ratio, time), the ablation behind the learned result")
@@ -400,7 +536,8 @@ def main():
             except Exception:
                 pass
     if args.only:
-        fn = {"c2": config_c2, "c3": config_c3, "c5": lambda R, d: config_c5(R, d, torch.cuda.current_stream(d))}
+        fn = {"c2": config_c2, "c3": config_c3, "c3_literal": config_c3_literal,
+              "c5": lambda R, d: config_c5(R, d, torch.cuda.current_stream(d))}
         print(json.dumps({args.only: fn[args.only](R, dev)}))
         return
     k, deg = 20, 2
@@ -549,19 +686,32 @@ def main():
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
         lap = O.abi.make_operator(1, [(0, O.abi.OP_LAP, 0, 0, 1.0)])
         t0 = time.perf_counter()
-        O.build(pts, pts[:ns], sample_adj, O.basis_struct(3, 2), lap, scaled=False, threads=threads)
+        _, _, w_or, _, _ = O.build(pts, pts[:ns], sample_adj, O.basis_struct(3, 2), lap, scaled=False,
+                                   threads=threads)
         cpu_s = time.perf_counter() - t0
         cpu = {"value": ns / cpu_s, "unit": "stencils/s", "cores": threads, "kind": "port",
                "sample": f"{ns} stencils (the first {ns} eval points of the same 4M cloud), "
                          f"C restatement of the reference arithmetic (rook LDL^T per stencil), OpenMP"}
+        # the GPU weights of each method on the same stencils against the oracle's (the
+        # reference arithmetic: uncentred, unscaled, FP64 rook LDL^T), per stencil
+        # max |w - w_oracle| / max |w_oracle|
+        wo = w_or[0].reshape(ns, k)
+        cpu["weight_error_vs_oracle"] = {}
+        for method in ("fp64", "fp32_refined", "learned"):
+            wg = ops[method].vals[0, :ns * k].cpu().numpy().reshape(ns, k)
+            rel = np.abs(wg - wo).max(1) / np.abs(wo).max(1)
+            cpu["weight_error_vs_oracle"][method] = {
+                "median": float(np.median(rel)), "p99.5": float(np.quantile(rel, 0.995)), "max": float(rel.max()),
+                "within_1e-10": float((rel <= 1e-10).mean())}
 
+    c4_poly = poly_residual(ops["fp64"], d_pts, d_pts, deg)
     configs = None
     if not args.no_configs:
         del ops, op
         torch.cuda.empty_cache()
         configs = {}
         for name, fn in (("c2", lambda: config_c2(R, dev)), ("c3", lambda: config_c3(R, dev)),
-                         ("c5", lambda: config_c5(R, dev, stream))):
+                         ("c3_literal", lambda: config_c3_literal(R, dev)), ("c5", lambda: config_c5(R, dev, stream))):
             t0 = time.time()
             configs[name] = fn()
             configs[name]["wall_s"] = time.time() - t0
@@ -590,6 +740,7 @@ def main():
         "equal_error_met": bool(ratio <= 1.05),
         "accepted_fraction": n_acc / args.n,
         "learned_sweep": sweep,
+        "poly_residual_fp64": c4_poly,
         "methods": results,
         "model": model_info,
         "knn_ms": knn_s * 1e3,
