@@ -703,6 +703,27 @@ def main():
             cpu["weight_error_vs_oracle"][method] = {
                 "median": float(np.median(rel)), "p99.5": float(np.quantile(rel, 0.995)), "max": float(rel.max()),
                 "within_1e-10": float((rel <= 1e-10).mean())}
+        # and against the truth (extended precision in the centred, radius-scaled frame,
+        # the tests' reference) on the first 50,000 of them, with the per-stencil bound
+        # max(1e-10, kappa_2 u) a backward-stable FP64 solve meets: at this density the
+        # uncentred reference arithmetic is itself the less accurate of the two
+        nt = min(ns, 50_000)
+        _, _, w_ld, _, _ = O.build(pts, pts[:nt], sample_adj[:nt], O.basis_struct(3, 2), lap, scaled=True,
+                                   long_double=True, threads=threads)
+        kap = O.condition_numbers(pts, pts[:nt], sample_adj[:nt], O.basis_struct(3, 2), lap)
+        bound = np.maximum(1e-10, np.where(np.isfinite(kap), kap, 1.0) * 2.0 ** -53)
+        wt = w_ld[0].reshape(nt, k)
+        cpu["weight_error_vs_truth"] = {"stencils": nt}
+        for method in ("fp64", "fp32_refined", "learned"):
+            wg = ops[method].vals[0, :nt * k].cpu().numpy().reshape(nt, k)
+            rel = np.abs(wg - wt).max(1) / np.abs(wt).max(1)
+            cpu["weight_error_vs_truth"][method] = {
+                "median": float(np.median(rel)), "p99.5": float(np.quantile(rel, 0.995)), "max": float(rel.max()),
+                "within_kappa_bound": float((rel <= bound).mean())}
+        rel = np.abs(wo[:nt] - wt).max(1) / np.abs(wt).max(1)
+        cpu["weight_error_vs_truth"]["reference_arithmetic"] = {
+            "median": float(np.median(rel)), "p99.5": float(np.quantile(rel, 0.995)), "max": float(rel.max()),
+            "within_kappa_bound": float((rel <= bound).mean())}
 
     c4_poly = poly_residual(ops["fp64"], d_pts, d_pts, deg)
     configs = None

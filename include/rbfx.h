@@ -240,8 +240,22 @@ typedef struct rbfx_spmv_args {
     int32_t mapping;             /* ABI 2: 0 automatic, 1 lanes per row reading HBM,
                                     2 staged through LDS (needs rows stored in processing
                                     order), 3 vector loads per lane (needs rows of one
-                                    length, no row_ptr, fewer than 8 fields);
+                                    length, no row_ptr, fewer than 8 fields), 4 source
+                                    tiles (ABI 3: needs the plan below, 8+ fields, one
+                                    value array);
                                     RBFX_EUNSUPPORTED when the mapping does not serve   */
+    /* ABI 3, optional: a source-tile plan from rbfx_source_tiles for these rows as
+       stored (all NULL / 0 without one). Tile t holds rows [t*tile_rows,
+       (t+1)*tile_rows); its distinct columns are tile_src[tile_src_off[t] ..
+       + tile_src_n[t]) and entry q's column is tile_src[tile_src_off[t] + tile_col[q]].
+       With mapping 4, an apply of 8 or more fields and one value array then copies
+       each tile's X rows into LDS once (not the automatic choice: slower than the
+       staged mapping on the applies measured).                                        */
+    int32_t tile_rows;
+    const int32_t* tile_src;
+    const int64_t* tile_src_off;
+    const int32_t* tile_src_n;
+    const uint16_t* tile_col;
 } rbfx_spmv_args;
 
 /* ---- entry points -------------------------------------------------------- */
@@ -304,6 +318,17 @@ int rbfx_spatial_order(const double* points, int64_t n, int32_t dim, int32_t* ou
 int rbfx_permute_rows(int64_t n_rows, int32_t k, const int32_t* row_order, const int32_t* col_idx,
                       const double* vals, int64_t nnz, int32_t n_valarrays, int32_t* out_col, double* out_vals,
                       void* stream);
+
+/* Source-tile plan (no reference counterpart) for fixed-length rows stored back to back
+ * (tile_rows * k <= 4096): per tile of tile_rows consecutive rows its distinct column
+ * indices, sorted (tile_src, at tile_src_off[t], tile_src_n[t] of them), and each
+ * entry's position in its tile's list (tile_col, nnz = n_rows * k entries). Blocking.
+ * Writes tile_src_n, tile_src_off (n_tiles = ceil(n_rows / tile_rows) each) and
+ * *n_src (their total); fills tile_src and tile_col only when tile_src != NULL and
+ * src_capacity >= *n_src (a call with tile_src == NULL sizes tile_src).            */
+int rbfx_source_tiles(int64_t n_rows, int32_t k, const int32_t* col_idx, int32_t tile_rows, int32_t* tile_src_n,
+                      int64_t* tile_src_off, uint16_t* tile_col, int32_t* tile_src, int64_t src_capacity,
+                      int64_t* n_src, void* stream);
 
 /* Number of stencils per status code from the last rbfx_build_weights on this
  * thread (counts[0..15]); lets the host raise the reference's exception types. */

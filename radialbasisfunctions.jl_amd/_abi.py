@@ -47,7 +47,7 @@ ST_TOO_LARGE = 10
 MAX_TERMS = 16
 MAX_COLS = 8
 MAX_N = 64
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 # surrogate network shape (must match csrc/mlp.h)
 MLP_LAYERS = 4
@@ -193,6 +193,11 @@ class SpmvArgs(C.Structure):
         ("rows_permuted", C.c_int32),
         ("max_row_len", C.c_int64),
         ("mapping", C.c_int32),
+        ("tile_rows", C.c_int32),
+        ("tile_src", C.c_void_p),
+        ("tile_src_off", C.c_void_p),
+        ("tile_src_n", C.c_void_p),
+        ("tile_col", C.c_void_p),
     ]
 
 

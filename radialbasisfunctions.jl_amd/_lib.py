@@ -54,6 +54,9 @@ def lib():
                 L.rbfx_build_weights_async.argtypes = [C.POINTER(abi.BuildArgs)]
                 L.rbfx_build_weights_noalloc.restype = C.c_int
                 L.rbfx_build_weights_noalloc.argtypes = [C.POINTER(abi.BuildArgs)]
+                L.rbfx_source_tiles.restype = C.c_int
+                L.rbfx_source_tiles.argtypes = [C.c_int64, C.c_int32, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_int64), C.c_void_p]
                 L.rbfx_spmv_async.restype = C.c_int
                 L.rbfx_spmv_async.argtypes = [C.POINTER(abi.SpmvArgs)]
                 L.rbfx_last_build_counts.restype = C.c_int
@@ -96,7 +99,7 @@ def exported_symbols():
             "rbfx_spmv", "rbfx_last_build_counts", "rbfx_last_build_timing", "rbfx_mlp_param_count", "rbfx_mlp_bf16_count",
             "rbfx_mlp_workspace_bytes", "rbfx_mlp_run", "rbfx_mlp_adam", "rbfx_mlp_sync_bf16",
             "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async",
-            "rbfx_build_weights_noalloc",
+            "rbfx_build_weights_noalloc", "rbfx_source_tiles",
             "rbfx_permute_rows", "rbfx_last_build_buckets", "rbfx_tune_bucket", "rbfx_last_build_stats",
             "rbfx_tune_save", "rbfx_tune_load", "rbfx_tune_reset"]
 
@@ -291,6 +294,27 @@ def spatial_order(points):
     with t.cuda.device(dev):
         _check(lib().rbfx_spatial_order(ptr(points), points.shape[0], points.shape[1], ptr(out), stream_ptr(dev)))
     return out
+
+
+def source_tiles(n_rows, k, col, tile_rows):
+    """Source-tile plan of fixed-length rows stored back to back (device col): per tile
+    of tile_rows rows its distinct columns and each entry's position among them.
+    Returns (tile_src_n int32, tile_src_off int64, tile_col int16 (read as uint16),
+    tile_src int32)."""
+    t = torch()
+    dev = col.device
+    n_tiles = (int(n_rows) + tile_rows - 1) // tile_rows
+    tn = t.empty(n_tiles, dtype=t.int32, device=dev)
+    toff = t.empty(n_tiles, dtype=t.int64, device=dev)
+    tcol = t.empty(int(n_rows) * int(k), dtype=t.int16, device=dev)
+    n_src = C.c_int64(0)
+    with t.cuda.device(dev):
+        _check(lib().rbfx_source_tiles(int(n_rows), int(k), ptr(col), int(tile_rows), ptr(tn), ptr(toff), ptr(tcol),
+                                       None, 0, C.byref(n_src), stream_ptr(dev)))
+        tsrc = t.empty(max(1, n_src.value), dtype=t.int32, device=dev)
+        _check(lib().rbfx_source_tiles(int(n_rows), int(k), ptr(col), int(tile_rows), ptr(tn), ptr(toff), ptr(tcol),
+                                       ptr(tsrc), n_src.value, C.byref(n_src), stream_ptr(dev)))
+    return tn, toff, tcol, tsrc
 
 
 def permute_rows(order, k, col, vals):

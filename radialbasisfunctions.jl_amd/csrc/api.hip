@@ -207,7 +207,20 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
     P.row_order = a->row_order;
     P.rows_permuted = a->rows_permuted;
     if (a->rows_permuted && !a->row_order) return set_err(RBFX_EINVAL, "rbfx_spmv: rows_permuted needs row_order");
-    if (a->mapping < 0 || a->mapping > 3) return set_err(RBFX_EINVAL, "rbfx_spmv: mapping must be 0, 1, 2 or 3");
+    if (a->mapping < 0 || a->mapping > 4) return set_err(RBFX_EINVAL, "rbfx_spmv: mapping must be 0..4");
+    P.tile_rows = a->tile_rows;
+    P.tile_src = a->tile_src;
+    P.tile_src_off = a->tile_src_off;
+    P.tile_src_n = a->tile_src_n;
+    P.tile_col = a->tile_col;
+    if (a->tile_col && (a->tile_rows < 1 || !a->tile_src || !a->tile_src_off || !a->tile_src_n))
+        return set_err(RBFX_EINVAL, "rbfx_spmv: a source-tile plan needs tile_rows, tile_src, tile_src_off and tile_src_n");
+    if (a->tile_col && a->n_fields < 8) {
+        P.tile_col = nullptr;  // (few fields: the plan does not apply; the other mappings run)
+    }
+    if (a->mapping == 4 && (!P.tile_col || a->n_valarrays != 1))
+        return set_err(RBFX_EUNSUPPORTED,
+                       "rbfx_spmv: the source-tile mapping needs a plan, 8 or more fields and one value array");
     if (a->max_row_len < 0) return set_err(RBFX_EINVAL, "rbfx_spmv: max_row_len must be >= 0");
     P.max_row = a->row_ptr ? a->max_row_len : a->k;
     P.mapping = a->mapping;
@@ -905,7 +918,7 @@ static int build_impl(const rbfx_build_args* a, Mode mode)
 
 extern "C" {
 
-int32_t rbfx_abi_version(void) { return 2; }
+int32_t rbfx_abi_version(void) { return 3; }
 
 const char* rbfx_last_error(void) { return g_err.c_str(); }
 
@@ -1243,6 +1256,44 @@ int rbfx_permute_rows(int64_t n_rows, int32_t k, const int32_t* row_order, const
     hipStream_t st = (hipStream_t)stream;
     HIPCHK(rbfx::launch_permute_rows(n_rows, k, row_order, col_idx, vals, nnz, n_valarrays, out_col, out_vals, st));
     HIPCHK(hipStreamSynchronize(st));
+    return RBFX_OK;
+}
+
+int rbfx_source_tiles(int64_t n_rows, int32_t k, const int32_t* col_idx, int32_t tile_rows, int32_t* tile_src_n,
+                      int64_t* tile_src_off, uint16_t* tile_col, int32_t* tile_src, int64_t src_capacity,
+                      int64_t* n_src, void* stream)
+{
+    if (!col_idx || !tile_src_n || !tile_src_off || !tile_col || !n_src)
+        return set_err(RBFX_EINVAL, "rbfx_source_tiles: null pointer argument");
+    if (n_rows < 0 || k < 1 || tile_rows < 1 || (int64_t)tile_rows * k > rbfx::kTileEntries)
+        return set_err(RBFX_EINVAL, "rbfx_source_tiles: need n_rows >= 0, k >= 1 and 1 <= tile_rows * k <= %d",
+                       rbfx::kTileEntries);
+    *n_src = 0;
+    if (n_rows == 0) return RBFX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n_tiles = (n_rows + tile_rows - 1) / tile_rows;
+    HIPCHK(rbfx::launch_source_tiles(n_rows, k, nullptr, col_idx, tile_rows, 0, tile_src_n, nullptr, nullptr, nullptr,
+                                     st));
+    std::vector<int32_t> cnt((size_t)n_tiles);
+    std::vector<int64_t> off((size_t)n_tiles);
+    HIPCHK(hipMemcpyAsync(cnt.data(), tile_src_n, sizeof(int32_t) * n_tiles, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    int64_t total = 0;
+    for (int64_t t = 0; t < n_tiles; ++t) {
+        off[(size_t)t] = total;
+        total += cnt[(size_t)t];
+    }
+    *n_src = total;
+    HIPCHK(hipMemcpyAsync(tile_src_off, off.data(), sizeof(int64_t) * n_tiles, hipMemcpyHostToDevice, st));
+    if (tile_src) {
+        if (src_capacity < total)
+            return set_err(RBFX_EINVAL, "rbfx_source_tiles: tile_src holds %lld entries, the plan needs %lld",
+                           (long long)src_capacity, (long long)total);
+        HIPCHK(rbfx::launch_source_tiles(n_rows, k, nullptr, col_idx, tile_rows, 1, tile_src_n, tile_src_off, tile_src,
+                                         tile_col, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    g_err.clear();
     return RBFX_OK;
 }
 

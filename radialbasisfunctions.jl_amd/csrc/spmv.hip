@@ -359,6 +359,253 @@ __global__ __launch_bounds__(256) void k_spmv_fields_staged(const SpmvDev P, int
     }
 }
 
+// Source-tile mapping (many fields, rows stored in processing order with a source-tile
+// plan): a workgroup owns one tile of consecutive rows. It copies into LDS, with
+// coalesced loads, the tile's distinct column ids, its entries (local column positions
+// and values) and then the X rows of those columns (every 16-byte piece a thread loads
+// issued before any is stored, so up to kTilePieces loads per thread are in flight); then
+// LPR lanes per row, FPL fields each, walk the row's entries entirely from LDS. A column
+// that several rows of the tile share (a source every ~60 targets of a regrid use)
+// crosses the cache hierarchy once per tile instead of once per entry. The entry order
+// of every row is the other mappings' own, so the sums are bit-identical to theirs.
+template <int NV, int LPR, int FPL, int U>
+__device__ __forceinline__ void tiled_rows(const SpmvDev& P, int64_t r0, int64_t r1, int64_t b0, const double* s_x,
+                                           const double* s_w, const uint16_t* s_l, int cap_e)
+{
+    const int F = P.n_fields;
+    const int fl = threadIdx.x % LPR;
+    const int f0 = fl * FPL;
+    const int64_t r = r0 + threadIdx.x / LPR;
+    if (r >= r1 || f0 >= F) return;
+    const int64_t row = P.row_order ? (int64_t)P.row_order[r] : r;
+    const int qb = (int)((P.row_ptr ? P.row_ptr[r] : r * (int64_t)P.k) - b0);
+    const int qe = (int)((P.row_ptr ? P.row_ptr[r + 1] : (r + 1) * (int64_t)P.k) - b0);
+    double acc[NV][FPL];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+#pragma unroll
+        for (int j = 0; j < FPL; ++j) acc[v][j] = 0.0;
+    for (int q0 = qb; q0 < qe; q0 += U) {
+        double xv[U][FPL], w[U][NV];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int q = q0 + u;
+            const bool ok = q < qe;
+            const int l = ok ? (int)s_l[q] : 0;
+            if constexpr (FPL == 2) {
+                const f64x2 t = *reinterpret_cast<const f64x2*>(s_x + l * F + f0);
+                xv[u][0] = ok ? t.x : 0.0;
+                xv[u][1] = ok ? t.y : 0.0;
+            } else {
+                xv[u][0] = ok ? s_x[l * F + f0] : 0.0;
+            }
+#pragma unroll
+            for (int v = 0; v < NV; ++v) w[u][v] = ok ? s_w[v * cap_e + q] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < NV; ++v)
+#pragma unroll
+                for (int j = 0; j < FPL; ++j) acc[v][j] += w[u][v] * xv[u][j];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+        double* yp = P.y + (int64_t)v * P.y_ld_val + row * P.y_ld_point + (int64_t)f0 * P.y_ld_field;
+        if constexpr (FPL == 2) {
+            f64x2 out = {P.alpha * acc[v][0], P.alpha * acc[v][1]};
+            if (P.beta != 0.0) {
+                const f64x2 old = *reinterpret_cast<const f64x2*>(yp);
+                out.x += P.beta * old.x;
+                out.y += P.beta * old.y;
+            }
+            *reinterpret_cast<f64x2*>(yp) = out;
+        } else {
+            const double out = P.alpha * acc[v][0];
+            *yp = (P.beta == 0.0) ? out : out + P.beta * *yp;
+        }
+    }
+}
+
+// X bytes a source-tile workgroup stages, and the 16-byte pieces of it per thread
+constexpr int kTileXBytes = 32768;
+constexpr int kTilePieces = kTileXBytes / (16 * 256);
+
+// LDS of a source-tile workgroup: X rows (cap_src x F doubles), values (NV x cap_e),
+// local column positions (cap_e) and column ids (cap_src)
+static size_t tiled_lds(int nv, int n_fields, int cap_src, int cap_e)
+{
+    return (size_t)cap_src * n_fields * 8 + (size_t)nv * cap_e * 8 + (size_t)((cap_e * 2 + 15) / 16 * 16) +
+           (size_t)cap_src * 4;
+}
+
+// (FPL = 2 needs fields_pair: X rows 16-byte aligned with the fields contiguous; a tile
+// whose distinct columns or entries exceed the LDS it was given gathers from X directly)
+// (one value array: with several, the kernel spills at any occupancy, and those applies
+// keep the other mappings. 3 waves per SIMD: three workgroups per CU, which their LDS
+// allows; left to itself the compiler spends 256 VGPRs on the unrolled copy and runs one.)
+template <int NV, int LPR, int FPL, int U>
+__global__ __attribute__((amdgpu_flat_work_group_size(256, 256), amdgpu_waves_per_eu(3))) void
+k_spmv_fields_tiled(
+    const SpmvDev P, int64_t n_tiles, int cap_src, int cap_e)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char s_raw[];
+    const int F = P.n_fields;
+    double* s_x = reinterpret_cast<double*>(s_raw);                                            // cap_src x F
+    double* s_w = s_x + (int64_t)cap_src * F;                                                  // NV x cap_e
+    uint16_t* s_l = reinterpret_cast<uint16_t*>(s_w + (int64_t)NV * cap_e);                     // cap_e
+    int32_t* s_src = reinterpret_cast<int32_t*>(reinterpret_cast<unsigned char*>(s_l) + (cap_e * 2 + 15) / 16 * 16);
+    constexpr int RP = 256 / LPR;  // rows per pass of the workgroup
+    const TileRange tr = xcd_tile_range(n_tiles);
+    for (int64_t t = tr.first; t < tr.end; t += tr.stride) {  // uniform over the workgroup
+        const int64_t r0 = t * P.tile_rows;
+        const int64_t r1 = r0 + P.tile_rows < P.n_rows ? r0 + P.tile_rows : P.n_rows;
+        const int64_t b0 = P.row_ptr ? P.row_ptr[r0] : r0 * (int64_t)P.k;
+        const int n_e = (int)((P.row_ptr ? P.row_ptr[r1] : r1 * (int64_t)P.k) - b0);
+        const int ns = P.tile_src_n[t];
+        if (ns > cap_src || n_e > cap_e) {
+            for (int64_t ra = r0; ra < r1; ra += RP)
+                fields_rows<NV, LPR, FPL, U>(P, ra, ra + RP < r1 ? ra + RP : r1, 0, P.col_idx, P.vals, P.nnz);
+            continue;  // LDS untouched in this tile
+        }
+        const int32_t* src = P.tile_src + P.tile_src_off[t];
+        for (int i = threadIdx.x; i < ns; i += 256) s_src[i] = src[i];
+        for (int i = threadIdx.x; i < n_e; i += 256) {
+            s_l[i] = P.tile_col[b0 + i];
+#pragma unroll
+            for (int v = 0; v < NV; ++v) s_w[v * cap_e + i] = __builtin_nontemporal_load(P.vals + (int64_t)v * P.nnz + b0 + i);
+        }
+        __syncthreads();
+        const int FP = FPL == 2 ? F >> 1 : F;  // pieces per X row (16 or 8 bytes each)
+        const int n_p = ns * FP;               // <= kTilePieces * 256 (cap_src is sized so)
+        if constexpr (FPL == 2) {
+            f64x2 tmp[kTilePieces];
+#pragma unroll
+            for (int p = 0; p < kTilePieces; ++p) {
+                const int i = threadIdx.x + p * 256;
+                if (i < n_p) {
+                    const int si = i / FP, f2 = i - si * FP;
+                    tmp[p] = *reinterpret_cast<const f64x2*>(P.x + (int64_t)s_src[si] * P.x_ld_point + 2 * f2);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < kTilePieces; ++p) {
+                const int i = threadIdx.x + p * 256;
+                if (i < n_p) {
+                    const int si = i / FP, f2 = i - si * FP;
+                    *reinterpret_cast<f64x2*>(s_x + si * F + 2 * f2) = tmp[p];
+                }
+            }
+        } else {
+            for (int i = threadIdx.x; i < n_p; i += 256) {
+                const int si = i / F, f = i - si * F;
+                s_x[si * F + f] = P.x[(int64_t)s_src[si] * P.x_ld_point + (int64_t)f * P.x_ld_field];
+            }
+        }
+        __syncthreads();
+        for (int64_t ra = r0; ra < r1; ra += RP)
+            tiled_rows<NV, LPR, FPL, U>(P, ra, ra + RP < r1 ? ra + RP : r1, b0, s_x, s_w, s_l, cap_e);
+        __syncthreads();  // the next tile overwrites the LDS copies
+    }
+}
+
+// One workgroup per tile: the tile's (column, entry) pairs sorted by column in LDS
+// (bitonic, NE a power of two >= the tile's entries; padding sorts last), the first of
+// every run of equal columns flagged, an inclusive scan of the flags giving each entry
+// its column's position among the tile's distinct columns.
+template <int NE>
+__global__ __launch_bounds__(256) void k_source_tiles(int64_t n_rows, int k, const int64_t* __restrict__ row_ptr,
+                                                      const int32_t* __restrict__ col, int tile_rows, int phase,
+                                                      int32_t* __restrict__ tile_n,
+                                                      const int64_t* __restrict__ tile_off,
+                                                      int32_t* __restrict__ tile_src, uint16_t* __restrict__ tile_col)
+{
+    __shared__ int32_t s_key[NE];
+    __shared__ uint16_t s_id[NE];
+    __shared__ int32_t s_inc[NE];
+    __shared__ int32_t s_part[256];
+    constexpr int PER = NE / 256;
+    const int64_t t = blockIdx.x;
+    const int64_t r0 = t * tile_rows;
+    const int64_t r1 = r0 + tile_rows < n_rows ? r0 + tile_rows : n_rows;
+    const int64_t b0 = row_ptr ? row_ptr[r0] : r0 * (int64_t)k;
+    const int64_t b1 = row_ptr ? row_ptr[r1] : r1 * (int64_t)k;
+    const int n_e = (int)(b1 - b0);  // <= NE: the host checks the longest tile
+    for (int i = threadIdx.x; i < NE; i += 256) {
+        s_key[i] = i < n_e ? col[b0 + i] : INT32_MAX;
+        s_id[i] = (uint16_t)i;
+    }
+    __syncthreads();
+    for (int size = 2; size <= NE; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < NE / 2; i += 256) {
+                const int lo = 2 * stride * (i / stride) + (i % stride);
+                const int hi = lo + stride;
+                const bool up = (lo & size) == 0;
+                const int32_t a = s_key[lo], b = s_key[hi];
+                if ((a > b) == up) {
+                    s_key[lo] = b;
+                    s_key[hi] = a;
+                    const uint16_t ia = s_id[lo];
+                    s_id[lo] = s_id[hi];
+                    s_id[hi] = ia;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // flags (first of a run of equal valid columns), scanned: each thread PER consecutive
+    int run = 0;
+    const int i0 = threadIdx.x * PER;
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int i = i0 + j;
+        run += (i < n_e && (i == 0 || s_key[i] != s_key[i - 1])) ? 1 : 0;
+        s_inc[i] = run;
+    }
+    s_part[threadIdx.x] = run;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int j = 0; j < 256; ++j) {
+            const int v = s_part[j];
+            s_part[j] = acc;
+            acc += v;
+        }
+    }
+    __syncthreads();
+    const int base = s_part[threadIdx.x];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) s_inc[i0 + j] += base;
+    __syncthreads();
+    if (phase == 0) {
+        if (threadIdx.x == 0) tile_n[t] = n_e > 0 ? s_inc[n_e - 1] : 0;
+        return;
+    }
+    const int64_t off = tile_off[t];
+    for (int i = threadIdx.x; i < n_e; i += 256) {
+        const int lid = s_inc[i] - 1;
+        tile_col[b0 + s_id[i]] = (uint16_t)lid;
+        if (i == 0 || s_key[i] != s_key[i - 1]) tile_src[off + lid] = s_key[i];
+    }
+}
+
+hipError_t launch_source_tiles(int64_t n_rows, int k, const int64_t* row_ptr, const int32_t* col, int tile_rows,
+                               int phase, int32_t* tile_n, const int64_t* tile_off, int32_t* tile_src,
+                               uint16_t* tile_col, hipStream_t s)
+{
+    const int64_t n_tiles = (n_rows + tile_rows - 1) / tile_rows;
+    if (n_tiles <= 0) return hipSuccess;
+    const int64_t most = row_ptr ? (int64_t)kTileEntries : (int64_t)tile_rows * k;  // (row_ptr: checked by the caller)
+    if (most <= 2048)
+        hipLaunchKernelGGL(k_source_tiles<2048>, dim3((unsigned)n_tiles), dim3(256), 0, s, n_rows, k, row_ptr, col,
+                           tile_rows, phase, tile_n, tile_off, tile_src, tile_col);
+    else
+        hipLaunchKernelGGL(k_source_tiles<kTileEntries>, dim3((unsigned)n_tiles), dim3(256), 0, s, n_rows, k, row_ptr,
+                           col, tile_rows, phase, tile_n, tile_off, tile_src, tile_col);
+    return hipGetLastError();
+}
+
 // LDS bytes of one staged workgroup
 static int64_t staged_lds(int nv, int G, int64_t max_row) { return (256 / G) * max_row * (8 * (int64_t)nv + 4); }
 constexpr int64_t kStagedLdsMax = 48 * 1024;
@@ -605,12 +852,40 @@ static hipError_t spmv_nv(const SpmvDev& P, double avg_row, hipStream_t s)
         if (P.n_fields <= 4) return spmv_staged<NV, 4, false>(P, s);
         return spmv_staged<NV, 8, false>(P, s);
     }
+    if (P.mapping == 4 && !(NV == 1 && P.tile_col && P.n_fields >= 8)) return hipErrorInvalidValue;
     if (P.n_fields >= 8) {
         int G = 8;
         while (G < P.n_fields) G <<= 1;
         if (G > 64) return hipErrorInvalidValue;
         const bool pair = fields_pair(P);
         const int lpr = fields_lpr(P);
+        // source tiles, when asked for: measured on the 16-field 8M-row regrid at 2.84 ms
+        // against 1.73 ms for the staged mapping below (each tile's staging latency and
+        // three LDS reads per entry and lane outweigh the 5.7x fewer X gathers), so the
+        // automatic choice stays the staged one
+        if constexpr (NV == 1) if (P.tile_col && P.mapping == 4) {
+            const int64_t n_tiles = (P.n_rows + P.tile_rows - 1) / P.tile_rows;
+            const int cap = kTileXBytes / (8 * P.n_fields);
+            const int cap_e = tile_entries_lds(NV);
+            const size_t lds = tiled_lds(NV, P.n_fields, cap, cap_e);
+#define RBFX_FTILED(LL, FF)                                                                                        \
+    {                                                                                                              \
+        hipLaunchKernelGGL((k_spmv_fields_tiled<NV, LL, FF, 4>), dim3((unsigned)pad8(n_tiles)), dim3(256), lds, s, P, \
+                           n_tiles, cap, cap_e);                                                                   \
+        return hipGetLastError();                                                                                  \
+    }
+            if (pair) {
+                if (lpr == 4) RBFX_FTILED(4, 2)
+                if (lpr == 8) RBFX_FTILED(8, 2)
+                if (lpr == 16) RBFX_FTILED(16, 2)
+                RBFX_FTILED(32, 2)
+            }
+            if (lpr == 8) RBFX_FTILED(8, 1)
+            if (lpr == 16) RBFX_FTILED(16, 1)
+            if (lpr == 32) RBFX_FTILED(32, 1)
+            RBFX_FTILED(64, 1)
+#undef RBFX_FTILED
+        }
         const int64_t cap = (256 / lpr) * P.max_row;
         const bool fstaged = spmv_staged_fits(P, NV);
         if (P.mapping == 2 && !fstaged) return hipErrorInvalidValue;

@@ -417,7 +417,9 @@ class RadialBasisOperator:
     scipy.sparse matrices (a tuple for rank >= 1)."""
 
     SPATIAL_ORDER_MIN = 65536  # below this the X vector sits in L2 anyway
-    SPMV_MAPPING = 0  # rbfx_spmv_args.mapping: 0 automatic, 1 lanes per row, 2 staged through LDS, 3 vector loads
+    SPMV_MAPPING = 0  # rbfx_spmv_args.mapping: 0 automatic, 1 lanes per row, 2 staged through LDS, 3 vector loads,
+    # 4 source tiles
+    SOURCE_TILE_FIELDS = 8  # mapping 4 (source tiles) serves applies of this many fields or more
 
     def __init__(self, L, data, eval_points=None, basis=None, k=None, adjl=None, hermite=None, device=None,
                  precision="fp64", refine_steps=4, accept_tol=1e-12, model=None, poly_deg_map=None, _weights=None):
@@ -468,6 +470,7 @@ class RadialBasisOperator:
         self._row_order = None
         self._max_row = None
         self._plan = None
+        self._tiles = None
         self._captured = False
         if _weights is None:
             self._build()
@@ -589,6 +592,7 @@ class RadialBasisOperator:
         self.n_cols = n_cols
         self._host_weights = None
         self._plan = None
+        self._tiles = None
 
     def update_weights(self):
         """update_weights!(op) (operators.jl:589-602)."""
@@ -611,6 +615,7 @@ class RadialBasisOperator:
         Y = t.empty((self.n_cols, self.n_eval, X.shape[1]), dtype=t.float64, device=self.dev)
         self._captured = True
         self._plan = None
+        self._tiles = None
         self._build()               # blocking warm-up: sizes the library's lists
         self._spmv(X, out=Y)        # warm-up: computes the row order
         t.cuda.synchronize(self.dev)
@@ -692,6 +697,22 @@ class RadialBasisOperator:
             s.col_idx = _lib.ptr(self._plan[0])
             s.vals = _lib.ptr(self._plan[1])
             s.rows_permuted = 1
+            if F >= self.SOURCE_TILE_FIELDS and self.n_cols == 1 and self.SPMV_MAPPING == 4:
+                # source tiles: a tile's distinct X rows are staged in LDS once; 32 rows per
+                # tile up to 16 fields (a Morton block of a 3-D regrid's 32 targets uses ~170
+                # sources, 32 KB of 16 fields), fewer beyond, within the entries a tile
+                # holds in LDS (1024 for one value array)
+                rows = 32 if F <= 16 else (16 if F <= 32 else 8)
+                while rows > 1 and rows * self.k > 1024:
+                    rows //= 2
+                if self._tiles is None or self._tiles[0] != rows:
+                    self._tiles = (rows,) + _lib.source_tiles(self.n_eval, self.k, self._plan[0], rows)
+                rows, tn, toff, tcol, tsrc = self._tiles
+                s.tile_rows = rows
+                s.tile_src = _lib.ptr(tsrc)
+                s.tile_src_off = _lib.ptr(toff)
+                s.tile_src_n = _lib.ptr(tn)
+                s.tile_col = _lib.ptr(tcol)
         return s, Y
 
     def __call__(self, x, y=None):
@@ -763,6 +784,8 @@ class RadialBasisOperator:
         new.L = L
         new.vals = vals
         new._host_weights = None
+        new._plan = None  # (the apply plan holds a copy of the values it was made from)
+        new._tiles = None
         return new
 
     def _check_compatible(self, other):

@@ -26,7 +26,7 @@ def test_library_exports_declared_symbols(RBF):
 def test_abi_version_without_gpu(RBF):
     lib = C.CDLL(str(ROOT / "radialbasisfunctions.jl_amd" / "librbfx.so"))
     lib.rbfx_abi_version.restype = C.c_int32
-    assert lib.rbfx_abi_version() == RBF._abi.ABI_VERSION == 2
+    assert lib.rbfx_abi_version() == RBF._abi.ABI_VERSION == 3
 
 
 def test_ctypes_mirror_sizes(RBF, O):

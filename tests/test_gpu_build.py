@@ -392,6 +392,87 @@ def test_built_operator_algebra(RBF, gpu):
         dxx + other
 
 
+def test_built_operator_algebra_after_apply(RBF, gpu):
+    """Algebra on operators whose apply plan (weights copied into processing order) exists:
+    the derived operator applies its own values, not the plan's copy of its parent's."""
+    import torch
+
+    pts = np.random.default_rng(6).random((70_000, 2))
+    dxx = RBF.partial(pts, 2, 1)
+    dyy = RBF.partial(pts, 2, 2, adjl=dxx.adj_idx.cpu().numpy())
+    fh = np.sin(3 * pts[:, 0]) * np.cos(2 * pts[:, 1])
+    f = torch.as_tensor(fh, device=gpu)
+    a, b = dxx(f), dyy(f)
+    assert dxx._plan is not None
+    # summation-order rounding of the k-term dot products (the parent's values applied in
+    # place of the derived ones would be off by O(1))
+    bound = 1e-13 * (abs(dxx.weights()) @ np.abs(fh) + abs(dyy.weights()) @ np.abs(fh)) * 2.5
+    for derived, ref in ((dxx + dyy, a + b), (2.5 * dxx, 2.5 * a), (dxx - dyy, a - b)):
+        got = derived(f)
+        assert np.all(np.abs((got - ref).cpu().numpy()) <= bound), derived
+
+
+def _tiles_numpy(col, k, rows):
+    """The source-tile plan restated: per tile of `rows` rows, its sorted distinct columns
+    and each entry's position among them."""
+    n = len(col) // k
+    out_n, out_src, out_col = [], [], np.empty(len(col), dtype=np.int64)
+    for t in range(0, n, rows):
+        seg = col[t * k:min(n, t + rows) * k]
+        u, inv = np.unique(seg, return_inverse=True)
+        out_n.append(len(u))
+        out_src.append(u)
+        out_col[t * k:t * k + len(seg)] = inv
+    return np.array(out_n), out_src, out_col
+
+
+@pytest.mark.parametrize("fields", [8, 16, 24])
+def test_source_tile_apply(RBF, gpu, fields):
+    """Many-field applies with a source-tile plan (a tile's distinct X rows staged in LDS):
+    the plan equals its numpy restatement; the product equals scipy's W @ X to rounding
+    and the staged many-field mapping's (same entry order) bit for bit; and a tile whose
+    distinct columns exceed its LDS (a scrambled adjacency) gathers directly."""
+    import torch
+
+    rng = np.random.default_rng(30 + fields)
+    src = rng.random((40_000, 3))
+    ev = rng.random((70_001, 3))
+    op = RBF.regrid(src, ev, basis=RBF.PHS(3, poly_deg=1), k=30, device=gpu)
+    X = torch.as_tensor(rng.standard_normal((len(src), fields)), device=gpu)
+    W = op.weights()
+    ref = W @ X.cpu().numpy()
+    bound = 1e-13 * (abs(W) @ np.abs(X.cpu().numpy()))
+    op.SPMV_MAPPING = 4
+    y_tiled = op._spmv(X)[0]
+    assert op._tiles is not None
+    rows, tn, toff, tcol, tsrc = op._tiles
+    pc = op._plan[0].cpu().numpy()
+    n_ref, src_ref, col_ref = _tiles_numpy(pc, 30, rows)
+    assert np.array_equal(tn.cpu().numpy(), n_ref)
+    offs = toff.cpu().numpy()
+    tsrc_h = tsrc.cpu().numpy()
+    assert np.array_equal(offs, np.concatenate([[0], np.cumsum(n_ref)[:-1]]))
+    for t in (0, 1, len(n_ref) // 2, len(n_ref) - 1):
+        assert np.array_equal(tsrc_h[offs[t]:offs[t] + n_ref[t]], src_ref[t])
+    assert np.array_equal(tcol.cpu().numpy().view(np.uint16).astype(np.int64), col_ref)
+    assert np.all(np.abs(y_tiled.cpu().numpy() - ref) <= bound)
+    op.SPMV_MAPPING = 2  # the staged mapping: the same per-row code, entries from HBM
+    y_staged = op._spmv(X)[0]
+    assert torch.equal(y_tiled, y_staged)
+    # scrambled columns: every tile's distinct sources exceed the LDS copy it may hold
+    # (30 consecutive entries of one random permutation per row: distinct columns, so the
+    # stencils stay nonsingular)
+    perm = rng.permutation(len(src))
+    adj = perm[(np.arange(len(ev))[:, None] * 30 + np.arange(30)) % len(src)].astype(np.int32)
+    op2 = RBF.regrid(src, ev, basis=RBF.PHS(3, poly_deg=1), adjl=adj, device=gpu)
+    W2 = op2.weights()
+    ref2 = W2 @ X.cpu().numpy()
+    op2.SPMV_MAPPING = 4
+    y2 = op2._spmv(X)[0].cpu().numpy()
+    assert (op2._tiles[1].cpu().numpy() > 32768 // (8 * fields)).any()
+    assert np.all(np.abs(y2 - ref2) <= 1e-13 * (abs(W2) @ np.abs(X.cpu().numpy())))
+
+
 def test_spatial_order_and_ordered_apply(RBF, gpu):
     """The Morton order is a permutation that keeps neighbours close, and the apply
     gives bit-identical results with and without it (it only changes which rows a
