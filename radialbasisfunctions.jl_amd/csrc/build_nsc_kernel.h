@@ -73,7 +73,8 @@ __device__ __forceinline__ double gmax(double v)
 // estimates (v_rsq_f64, v_rcp_f64) refined by Newton steps. Their rounding differs from
 // the correctly rounded operations by at most an ulp or so, a relative perturbation of
 // the assembled system of order 1e-16, inside the conditioning bound the build meets.
-// nsc_sqrt(0) = 0 (the PHS diagonal); arguments are finite and not negative.
+// nsc_sqrt(0) = 0 (the PHS diagonal: the estimate is taken at max(x, 1e-300), so
+// x = 0 gives s = 0 through every step); arguments are finite and not negative.
 __device__ __forceinline__ double nsc_rsq(double x)
 {
     double y = __builtin_amdgcn_rsq(x);
@@ -84,13 +85,12 @@ __device__ __forceinline__ double nsc_rsq(double x)
 }
 __device__ __forceinline__ double nsc_sqrt(double x)
 {
-    const double y = __builtin_amdgcn_rsq(x);
+    const double y = __builtin_amdgcn_rsq(fmax(x, 1e-300));
     double s = x * y, h = 0.5 * y;
     const double r = fma(-s, h, 0.5);
     s = fma(s, r, s);
     h = fma(h, r, h);
-    s = fma(fma(-s, s, x), h, s);
-    return x > 0.0 ? s : 0.0;
+    return fma(fma(-s, s, x), h, s);
 }
 __device__ __forceinline__ double nsc_rcp(double d)
 {
@@ -313,13 +313,14 @@ k_nsc(const BuildDev P)
         }
     }
 
-    // ---- monomials of the own points (graded order; any order spans the same space)
+    // ---- monomials of the own points (graded order, the table's: the exponents are
+    // compile-time constants here, so no per-lane selects)
     double p[SL][MMAX];
 #pragma unroll
     for (int s = 0; s < SL; ++s)
 #pragma unroll
         for (int a = 0; a < MMAX; ++a)
-            p[s][a] = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
+            p[s][a] = (is_pt[s] && a < M) ? mono_graded<DIM>(a, xi[s]) : 0.0;
     __syncthreads();
 
     // ---- collocation rows Phi[row][j] = phi(|xi_row - xi_j|). Rows and columns past the
@@ -687,6 +688,17 @@ k_nsc(const BuildDev P)
         int settled = 0;
 #pragma unroll 1
         for (int it = 0; it < NREF; ++it) {
+            // (the residual evaluates the monomials again from a copy of the coordinates the
+            // compiler must treat as new: reusing the first evaluation would hold them in
+            // registers through the whole factorisation)
+            double xr[SL][DIM];
+#pragma unroll
+            for (int s = 0; s < SL; ++s)
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    xr[s][d] = xi[s][d];
+                    asm volatile("" : "+v"(xr[s][d]));
+                }
             double r[SL][RC];
 #pragma unroll
             for (int s = 0; s < SL; ++s) {
@@ -695,7 +707,7 @@ k_nsc(const BuildDev P)
 #pragma unroll
                 for (int a = 0; a < MMAX; ++a) {
                     if (a < m_wave) {
-                        const double pa = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
+                        const double pa = (is_pt[s] && a < M) ? mono_graded<DIM>(a, xr[s]) : 0.0;
 #pragma unroll
                         for (int c = 0; c < RC; ++c) r[s][c] = fma(-pa, S.lam[c][a], r[s][c]);
                     }
@@ -739,7 +751,7 @@ k_nsc(const BuildDev P)
                     double pa[SL];
 #pragma unroll
                     for (int s = 0; s < SL; ++s)
-                        pa[s] = (is_pt[s] && a < M) ? mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi[s]) : 0.0;
+                        pa[s] = (is_pt[s] && a < M) ? mono_graded<DIM>(a, xr[s]) : 0.0;
 #pragma unroll
                     for (int c = 0; c < RC; ++c) {
                         if (c < n_cols) {

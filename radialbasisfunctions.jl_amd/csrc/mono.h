@@ -22,6 +22,42 @@ __device__ __forceinline__ double ipow(double x, int p)
     return v;
 }
 
+// Exponent of x_d in the a-th monomial of the graded order fill_mono_table writes (total
+// degree ascending; within one degree, the first exponent descending, then the second):
+// a compile-time constant wherever a and d are, so a monomial's value is a fixed
+// sequence of multiplies.
+__host__ __device__ constexpr int graded_exp(int dim, int a, int d)
+{
+    int c = 0;
+    for (int tot = 0; tot <= kMaxPolyDeg; ++tot) {
+        if (dim == 1) {
+            if (c++ == a) return d == 0 ? tot : 0;
+        } else if (dim == 2) {
+            for (int x = tot; x >= 0; --x)
+                if (c++ == a) return d == 0 ? x : (d == 1 ? tot - x : 0);
+        } else {
+            for (int x = tot; x >= 0; --x)
+                for (int y = tot - x; y >= 0; --y)
+                    if (c++ == a) return d == 0 ? x : (d == 1 ? y : tot - x - y);
+        }
+    }
+    return 0;
+}
+
+// value of the a-th graded monomial at x (a a compile-time constant after unrolling)
+template <int DIM>
+__device__ __forceinline__ double mono_graded(const int a, const double (&x)[DIM])
+{
+    double v = 1.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+#pragma unroll
+        for (int q = 0; q < kMaxPolyDeg; ++q)
+            if (q < graded_exp(DIM, a, d)) v *= x[d];
+    }
+    return v;
+}
+
 // operator term `kind` applied to the monomial x^e
 template <int DIM>
 __device__ __forceinline__ double mono_op(const int (&e)[3], int kind, int d1, int d2, const double (&x)[DIM])
