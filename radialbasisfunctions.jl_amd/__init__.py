@@ -17,5 +17,7 @@ from .operators import (AbstractOperator, RadialBasisOperator, Laplacian, Partia
                         reorder_points,
                         weights, update_weights, is_cache_valid, print_op)
 from .learned import LearnedModel
+from . import functors
+from .notation import operator
 
 __all__ = [n for n in dir() if not n.startswith("_")]

@@ -35,6 +35,13 @@ class AbstractRadialBasis:
     def descriptor(self):
         return abi.Basis(self.kind, self.poly_deg, float(self.eps))
 
+    def __call__(self, x, xi):
+        """φ(‖x - xᵢ‖) on the host (polyharmonic_spline.jl:50-51, inverse_multiquadric.jl:33-34,
+        gaussian.jl:30-31); inside a custom closure being lowered, the value term."""
+        from .functors import basis_value
+
+        return basis_value(self, x, xi)
+
     def __eq__(self, other):
         return type(self) is type(other) and self.poly_deg == other.poly_deg and self.eps == other.eps
 

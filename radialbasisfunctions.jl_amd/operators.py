@@ -19,6 +19,7 @@ import numpy as np
 
 from . import _abi as abi
 from . import _lib
+from . import functors
 from .basis import AbstractRadialBasis, PHS, n_monomials, is_dirichlet
 from .errors import ArgumentError, DimensionMismatch, SingularException
 
@@ -30,9 +31,11 @@ class AbstractOperator:
     """rank: tensor rank the operator adds to its output (operators.jl:20)."""
 
     rank = 0
+    __array_ufunc__ = None  # numpy scalars defer to the operators below
 
-    def lower(self, dim):
-        """Return (n_cols, [(col, kind, dim1, dim2, coeff), ...]) with 0-based dims."""
+    def lower(self, dim, basis=None):
+        """Return (n_cols, [(col, kind, dim1, dim2, coeff), ...]) with 0-based dims; `basis`
+        is the radial basis of the build (custom closures are lowered against it)."""
         raise NotImplementedError
 
     def __call__(self, data, **kw):
@@ -76,7 +79,7 @@ def _terms(op):
 class Laplacian(AbstractOperator):
     """∇² (laplacian.jl:6-7); for PHS the reference's 3-D constants are kept."""
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         return 1, [(0, abi.OP_LAP, 0, 0, 1.0)]
 
     def __repr__(self):
@@ -89,7 +92,7 @@ class Partial(AbstractOperator):
     def __init__(self, order, dim):
         self.order, self.dim = int(order), int(dim)
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         if self.order not in (1, 2):
             raise ArgumentError("Only first and second order derivatives are supported. "
                                 "Use the custom operator for higher orders.")
@@ -108,7 +111,7 @@ class MixedPartial(AbstractOperator):
     def __init__(self, dim1, dim2):
         self.dim1, self.dim2 = int(dim1), int(dim2)
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         for d in (self.dim1, self.dim2):
             if not 1 <= d <= dim:
                 raise ArgumentError(f"dimension {d} out of range for {dim}-D data")
@@ -128,7 +131,7 @@ class Jacobian(AbstractOperator):
     def __init__(self, dim):
         self.dim = int(dim)
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         if dim != self.dim:
             raise ArgumentError(f"Jacobian{{{self.dim}}} applied to {dim}-D data")
         return dim, [(c, abi.OP_D1, c, 0, 1.0) for c in range(dim)]
@@ -150,7 +153,7 @@ class Hessian(AbstractOperator):
     def __init__(self, dim):
         self.dim = int(dim)
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         if dim != self.dim:
             raise ArgumentError(f"Hessian{{{self.dim}}} applied to {dim}-D data")
         return len(_sym_pairs(dim)), [
@@ -163,7 +166,7 @@ class Hessian(AbstractOperator):
 class Regrid(AbstractOperator):
     """Interpolation to other points: Regrid()(b) = b (regridding.jl:266-267)."""
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         return 1, [(0, abi.OP_VALUE, 0, 0, 1.0)]
 
     def __repr__(self):
@@ -173,7 +176,7 @@ class Regrid(AbstractOperator):
 class Identity(AbstractOperator):
     """Identity (operator_algebra.jl:83-85)."""
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         return 1, [(0, abi.OP_VALUE, 0, 0, 1.0)]
 
     def __repr__(self):
@@ -181,12 +184,18 @@ class Identity(AbstractOperator):
 
 
 class ScaledOperator(AbstractOperator):
+    """α · op (operator_algebra.jl:120-150)."""
+
     def __init__(self, alpha, op):
         self.alpha, self.op = alpha, op
         self.rank = op.rank
 
-    def lower(self, dim):
-        n, terms = self.op.lower(dim)
+    @property
+    def α(self):
+        return self.alpha
+
+    def lower(self, dim, basis=None):
+        n, terms = self.op.lower(dim, basis)
         return n, [(c, k, d1, d2, self.alpha * w) for (c, k, d1, d2, w) in terms]
 
     def __repr__(self):
@@ -200,8 +209,8 @@ class SumOperator(AbstractOperator):
         self.ops = tuple(ops)
         self.rank = self.ops[0].rank
 
-    def lower(self, dim):
-        lowered = [op.lower(dim) for op in self.ops]
+    def lower(self, dim, basis=None):
+        lowered = [op.lower(dim, basis) for op in self.ops]
         ns = {n for n, _ in lowered}
         if len(ns) != 1:
             raise ArgumentError("Cannot sum gradient-family operators with different dimensions.")
@@ -228,10 +237,10 @@ class Stacked(AbstractOperator):
             raise ArgumentError("Stacked takes operators of rank 0 or 1")
         self.ops = ops
 
-    def lower(self, dim):
+    def lower(self, dim, basis=None):
         n, terms = 0, []
         for o in self.ops:
-            nc, t = o.lower(dim)
+            nc, t = o.lower(dim, basis)
             terms += [(c + n, kind, d1, d2, coeff) for c, kind, d1, d2, coeff in t]
             n += nc
         return n, terms
@@ -241,20 +250,34 @@ class Stacked(AbstractOperator):
 
 
 class Custom(AbstractOperator):
-    """A custom operator given as an operator-algebra tree.
+    """Custom{N}(ℒ) (custom.jl:10-14): ℒ maps a basis to a callable (x, xᵢ) -> value, or to
+    a tuple of them for rank 1, written with basis(x, xᵢ) and the functors of
+    `functors` (d, d2, dmixed, lap); an operator-algebra tree is accepted too.
 
-    The reference also accepts an arbitrary Julia closure (custom.jl:10-14); a closure
-    cannot be lowered to device terms, so only AbstractOperator trees are accepted."""
+    A closure is lowered against the build's basis to the device term table by tracing
+    (functors.lower_closure); closures that are not constant-coefficient combinations
+    of those terms raise ArgumentError. Calling a Custom on a basis returns ℒ(basis)."""
 
     def __init__(self, op, rank=None):
-        if not isinstance(op, AbstractOperator):
-            raise ArgumentError("custom operators on the device must be built from the operator algebra "
-                                "(Laplacian, Partial, MixedPartial, Jacobian, Identity, sums and scalings)")
-        self.op = op
-        self.rank = op.rank if rank is None else int(rank)
+        if isinstance(op, AbstractOperator):
+            self.op, self.fn = op, None
+            self.rank = op.rank if rank is None else int(rank)
+        elif callable(op):
+            self.op, self.fn = None, op
+            self.rank = functors.infer_rank(op) if rank is None else int(rank)
+        else:
+            raise ArgumentError(f"custom takes a closure basis -> (x, xi) -> value or an operator, "
+                                f"got {type(op).__name__}")
 
-    def lower(self, dim):
-        return self.op.lower(dim)
+    def __call__(self, data, **kw):
+        if isinstance(data, AbstractRadialBasis):
+            return self.fn(data) if self.fn is not None else self.op
+        return RadialBasisOperator(self, data, **kw)
+
+    def lower(self, dim, basis=None):
+        if self.fn is None:
+            return self.op.lower(dim, basis)
+        return functors.lower_closure(self.fn, basis if basis is not None else PHS(3, poly_deg=2), dim, self.rank)
 
     def __repr__(self):
         return "Custom Operator"
@@ -271,6 +294,17 @@ def autoselect_k(data, basis):
     N, d = shape[0], (shape[1] if len(shape) > 1 else 1)
     m = basis.poly_deg
     return min(N, max(2 * n_monomials(d, m) if m >= 0 else 0, 2 * d + 1))
+
+
+def _float_type(x):
+    """"float32" for single-precision points (a Float32 element type in the reference), else "float64"."""
+    t = _lib.torch()
+    if isinstance(x, t.Tensor):
+        return "float32" if x.dtype == t.float32 else "float64"
+    dt = getattr(x, "dtype", None)
+    if dt is None and isinstance(x, (list, tuple)) and len(x):
+        dt = getattr(x[0], "dtype", None)
+    return "float32" if dt is not None and np.dtype(dt) == np.float32 else "float64"
 
 
 def _as_points(x):
@@ -433,6 +467,9 @@ class RadialBasisOperator:
         if not isinstance(self.basis, AbstractRadialBasis):
             raise ArgumentError("basis must be a radial basis (PHS, IMQ or Gaussian)")
         self.dev = _lib.device_for(device)
+        # Float32 points give Float32 weights and outputs (test/float32.jl:11-32): the system
+        # is still assembled and solved in FP64, and the weights are rounded to FP32 once
+        self.dtype = _float_type(data)
         self.data = _as_points(data)
         self.eval_points = self.data if eval_points is None else _as_points(eval_points)
         self.dim = self.data.shape[1]
@@ -494,7 +531,7 @@ class RadialBasisOperator:
     # ---------------------------------------------------------------- build
     def _build(self):
         t = _lib.torch()
-        n_cols, terms = self.L.lower(self.dim)
+        n_cols, terms = self.L.lower(self.dim, self.basis)
         op = abi.make_operator(n_cols, terms)
         a = abi.BuildArgs()
         a.dim = self.dim
@@ -584,6 +621,8 @@ class RadialBasisOperator:
             if rc == abi.RBFX_EUNSUPPORTED:
                 raise NotImplementedError(msg)
             raise ArgumentError(msg)
+        if self.dtype == "float32":
+            vals.copy_(vals.to(t.float32))  # the applied weights are the FP32 weights
         self.row_ptr = row_ptr
         self._max_row = None
         self._adj_flat = self.adj_idx.reshape(-1)
@@ -604,11 +643,14 @@ class RadialBasisOperator:
         a = self._args
         a.stream = _lib.stream_ptr(self.dev)
         _lib.build_weights_async(a, self.dev)
+        if self.dtype == "float32":
+            self.vals.copy_(self.vals.to(_lib.torch().float32))
 
     def capture(self, x):
         """Record weight build + apply of the field x (n_data,) or (n_data, F) in a HIP
         graph. Returns a GraphedOperator whose replay() rebuilds the weights from the
-        current coordinates and applies them to its `x` buffer into its `y` buffer."""
+        current coordinates and applies them to its `x` buffer into its `y` buffer
+        (float64 buffers; a Float32 operator's replay applies its FP32-rounded weights)."""
         t = _lib.torch()
         xd = _to_device(x, self.dev, t.float64)
         X = (xd[:, None] if xd.ndim == 1 else xd.reshape(xd.shape[0], -1)).clone().contiguous()
@@ -640,7 +682,7 @@ class RadialBasisOperator:
             ptr = (self.row_ptr.cpu().numpy() if self.row_ptr is not None
                    else np.arange(self.n_eval + 1, dtype=np.int64) * self.k)
             col = self.col.cpu().numpy()
-            vals = self.vals.cpu().numpy()
+            vals = self.vals.cpu().numpy().astype(self.dtype, copy=False)
             # each matrix owns its index arrays: scipy may canonicalise them in place
             mats = tuple(sp.csr_matrix((vals[c], col.copy(), ptr.copy()), shape=(self.n_eval, self.n_data))
                          for c in range(self.n_cols))
@@ -739,6 +781,8 @@ class RadialBasisOperator:
                 full[..., i, j] = Yp[c]
                 full[..., j, i] = Yp[c]
             res = full
+        if self.dtype == "float32":
+            res = res.to(t.float32)
         if y is not None:
             if isinstance(y, t.Tensor):
                 y.copy_(res)
@@ -773,7 +817,7 @@ class RadialBasisOperator:
         return (self.n_eval, self.n_data)
 
     def __repr__(self):
-        return (f"RadialBasisOperator\n├─Operator: {print_op(self.L)}\n├─Data type: float64\n"
+        return (f"RadialBasisOperator\n├─Operator: {print_op(self.L)}\n├─Data type: {self.dtype}\n"
                 f"├─Number of points: {self.n_data}\n├─Stencil size: {self.k}\n"
                 f"└─Basis: {type(self.basis).__name__} with degree {self.basis.poly_deg} polynomial augmentation")
 
@@ -782,6 +826,8 @@ class RadialBasisOperator:
         new = object.__new__(RadialBasisOperator)
         new.__dict__.update(self.__dict__)
         new.L = L
+        if self.dtype == "float32":
+            vals = vals.to(_lib.torch().float32).to(vals.dtype)
         new.vals = vals
         new._host_weights = None
         new._plan = None  # (the apply plan holds a copy of the values it was made from)
@@ -883,5 +929,9 @@ def regrid(data, eval_points, basis=None, **kw):
 
 
 def custom(data, op, basis=None, rank=None, **kw):
-    return RadialBasisOperator(op if isinstance(op, AbstractOperator) and rank is None else Custom(op, rank),
-                               data, basis=basis, **kw)
+    """custom(data, ℒ; rank, basis, ...) and custom(data, op::AbstractOperator) (custom.jl:57-69):
+    ℒ a closure basis -> (x, xᵢ) -> value (rank inferred as in _infer_rank when omitted),
+    or an operator-algebra tree, which is built as given (its rank is its own)."""
+    if isinstance(op, AbstractOperator):
+        return RadialBasisOperator(op, data, basis=basis, **kw)
+    return RadialBasisOperator(Custom(op, rank), data, basis=basis, **kw)

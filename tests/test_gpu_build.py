@@ -392,6 +392,48 @@ def test_built_operator_algebra(RBF, gpu):
         dxx + other
 
 
+def test_built_operator_algebra_against_oracle(RBF, O, gpu):
+    """Algebra on device weights (operator_algebra.jl:245-309: α*A + β*B with no
+    re-collocation) against the same combination of the oracle's extended-precision
+    weights. Each operand is first held to its per-stencil parity bound (max(1e-10, κ₂u)
+    relative, check_against_truth); the combination may then differ from the oracle's
+    by |α| and |β| times those errors plus the rounding of the combination itself, and
+    the device result equals the host's α*A + β*B of the same device weights bit for bit."""
+    pts = np.random.default_rng(9).random((3000, 2))
+    b = O.basis_struct(3, 2)
+    lap = RBF.laplacian(pts, k=20)
+    dxx = RBF.partial(pts, 2, 1, k=20)
+    adj = lap.adj_idx.cpu().numpy()
+    assert np.array_equal(adj, dxx.adj_idx.cpu().numpy())
+    dev, ref, err = {}, {}, {}
+    for name, opr, terms in (("lap", lap, [(0, O.abi.OP_LAP, 0, 0, 1.0)]),
+                             ("dxx", dxx, [(0, O.abi.OP_D2, 0, 0, 1.0)])):
+        dev[name] = opr.vals.cpu().numpy()
+        err[name], _ = check_against_truth(O, dev[name], pts, pts, adj, b, _op(O, 1, terms), 20)
+        _, _, w, _, rc = O.build(pts, pts, adj, b, _op(O, 1, terms), scaled=True, long_double=True)
+        assert rc == 0
+        ref[name] = w[0].reshape(-1, 20)
+        dev[name] = dev[name][0].reshape(-1, 20)
+    fh = np.sin(3 * pts[:, 0]) * np.cos(2 * pts[:, 1])
+    for alpha, beta in ((2.5, -0.5), (1.0, 1.0), (-1.0, 3.0)):
+        comb = alpha * lap + beta * dxx
+        got = comb.vals.cpu().numpy()[0].reshape(-1, 20)
+        assert np.array_equal(got, alpha * dev["lap"] + beta * dev["dxx"])
+        want = alpha * ref["lap"] + beta * ref["dxx"]
+        tol = (abs(alpha) * err["lap"] * np.abs(ref["lap"]).max(1) + abs(beta) * err["dxx"] * np.abs(ref["dxx"]).max(1)
+               + 4 * U64 * (np.abs(alpha * ref["lap"]) + np.abs(beta * ref["dxx"])).max(1))
+        assert np.all(np.abs(got - want).max(1) <= 1.01 * tol + 1e-300)
+        # applied: the combined weights against the oracle's combination, plus the apply's rounding
+        y = comb(fh)
+        y_want = (want * fh[adj]).sum(1)
+        bound = 20 * tol * np.abs(fh).max() + 1e-13 * (np.abs(want) * np.abs(fh[adj])).sum(1)
+        assert np.all(np.abs(y - y_want) <= bound)
+    neg = -lap
+    assert np.array_equal(neg.vals.cpu().numpy(), -lap.vals.cpu().numpy())
+    half = lap / 2
+    assert np.array_equal(half.vals.cpu().numpy(), lap.vals.cpu().numpy() * 0.5)
+
+
 def test_built_operator_algebra_after_apply(RBF, gpu):
     """Algebra on operators whose apply plan (weights copied into processing order) exists:
     the derived operator applies its own values, not the plan's copy of its parent's."""
