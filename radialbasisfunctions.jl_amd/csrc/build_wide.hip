@@ -1,20 +1,20 @@
 // Fused build for stencil systems of 65..RBFX_MAX_N rows (k up to 60 with a degree-3
 // polynomial gives 70): one wavefront per stencil, the augmented system [A | b] in LDS
-// (row-major, leading dimension lda >= n + RC, odd), lane li owning rows li and li+64.
+// (row-major, odd leading dimension lda >= n + RC rounded up to 8), lane li owning rows li
+// and li + 64.
 //
 // The arithmetic is that of k_build (build_kernel.h): centred, radius-scaled
 // saddle-point system assembled in FP64, rook-pivoted LDL^T (pivot tests of Ashcraft,
-// Grimes & Lewis) with rows that never move, the trailing update applied to every
-// column of the still-active rows, and a back substitution over the rows frozen at
-// their pivot step. Only the storage differs: the rows live in LDS instead of
-// registers, so a pivot-row read is an LDS load and the pivot search reads the
-// column entries of the lane's own two rows.
+// Grimes & Lewis) with rows that never move, the trailing update applied to the
+// still-active rows, and a back substitution over the rows frozen at their pivot step.
+// Only the storage differs: the rows live in LDS instead of registers, so a pivot-row
+// read is an LDS load and the pivot search reads the column entries of the lane's own
+// rows. The trailing update runs eight columns per LDS round trip.
 #include "build_kernel.h"
 
 namespace rbfx {
 
-constexpr int kWideRows = RBFX_MAX_N;  // <= 128: two rows per lane
-static_assert(kWideRows <= 128, "two rows per lane");
+static_assert(RBFX_MAX_N <= 128, "at most two rows per lane");
 
 __device__ __forceinline__ uint32_t row_key(double v, int row)
 {
@@ -22,15 +22,17 @@ __device__ __forceinline__ uint32_t row_key(double v, int row)
     return (hi & ~127u) | (uint32_t)(127 - row);
 }
 
-template <int DIM, int RC>
+// TWO: rows per lane (1 for systems of up to 64 rows, 2 up to 128)
+template <int DIM, int RC, int TWO>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wide(const BuildDev P, int rows_cap,
                                                                                   int lda)
 {
+    constexpr int ROWS = 64 * TWO;
     extern __shared__ __attribute__((aligned(16))) double s_A[];  // rows_cap x lda
-    __shared__ double s_xi[kWideRows][DIM];
-    __shared__ double s_ba[kWideRows], s_bb[kWideRows], s_bn[kWideRows][DIM];
-    __shared__ int s_bt[kWideRows];
-    __shared__ int s_piv[kWideRows], s_blk[kWideRows];
+    __shared__ double s_xi[ROWS][DIM];
+    __shared__ double s_ba[ROWS], s_bb[ROWS], s_bn[ROWS][DIM];
+    __shared__ int s_bt[ROWS];
+    __shared__ int s_piv[ROWS], s_blk[ROWS];
     __shared__ int s_mono[kMaxMono][3];
     __shared__ TermDev s_terms[RBFX_MAX_TERMS];
 
@@ -60,16 +62,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     }
     const int n = M + kk;
 
-    // ---- the stencil points this lane loads: q = L and q = L + 64
+    // ---- the stencil points this lane loads: q = L (and q = L + 64)
     double xe[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) xe[d] = P.eval[e * DIM + d];
-    int gidx[2] = {0, 0};
-    double D[2][DIM];
+    int gidx[TWO] = {};
+    double D[TWO][DIM];
     double r2max = 0.0;
-    bool ib[2] = {false, false};
+    bool ib[TWO] = {};
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TWO; ++t) {
         const int q = L + 64 * t;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) D[t][d] = 0.0;
@@ -86,7 +88,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         }
     }
     // stencil classification (types.jl:208-223)
-    const bool hermite = P.is_boundary && (__ballot(ib[0] || ib[1]) != 0ull);
+    const bool hermite = P.is_boundary && (__ballot(ib[0] || ib[TWO - 1]) != 0ull);
     // a Dirichlet eval node is an identity row whatever its neighbours are (the row
     // length count_nonzeros gives it, execution.jl:39-69)
     if (P.is_boundary && e < P.n_data && P.is_boundary[e]) {
@@ -104,9 +106,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     double h = sqrt(group_max<64>(r2max));
     if (!(h > 0.0)) h = 1.0;
     const double hinv = 1.0 / h;
-    unsigned long long at_eval[2];
+    unsigned long long at_eval[TWO];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TWO; ++t) {
         const int q = L + 64 * t;
         bool eq = false;
         if (q < kk) {
@@ -134,11 +136,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     }
     int eval_local = 0;
     if (hermite) {
-        if (!at_eval[0] && !at_eval[1]) {
+        const unsigned long long ae1 = TWO == 2 ? at_eval[TWO - 1] : 0ull;
+        if (!at_eval[0] && !ae1) {
             if (L == 0 && P.status) P.status[e] = RBFX_ST_EVAL_NOT_FOUND;
             return;
         }
-        eval_local = at_eval[0] ? __builtin_ctzll(at_eval[0]) : 64 + __builtin_ctzll(at_eval[1]);
+        eval_local = at_eval[0] ? __builtin_ctzll(at_eval[0]) : 64 + __builtin_ctzll(ae1);
     }
     __syncthreads();
 
@@ -146,9 +149,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     B.kind = P.basis_kind;
     B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
 
-    // ---- assembly (FP64): rows i = L, L + 64; columns 0..n-1, then the RHS at n..n+RC-1
+    // ---- assembly (FP64): rows i = L (and L + 64); columns 0..n-1, then the RHS at n..n+RC-1
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TWO; ++t) {
         const int i = L + 64 * t;
         if (i >= n) continue;
         double* Ai = s_A + (int64_t)i * lda;
@@ -264,9 +267,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
 
     // ---- factorisation with rook pivoting; rows never move, pivoted rows freeze
     const double alpha = 0.6403882032022076;  // (1 + sqrt(17)) / 8
-    const int ncol = n + RC;
-    bool act[2] = {L < n, L + 64 < n};
-    int myslot[2] = {1 << 20, 1 << 20};
+    bool act[TWO];
+    int myslot[TWO];
+#pragma unroll
+    for (int t = 0; t < TWO; ++t) {
+        act[t] = L + 64 * t < n;
+        myslot[t] = 1 << 20;
+    }
     int slot = 0;
     bool failed = false;
     auto A = [&](int i, int j) -> double { return s_A[(int64_t)i * lda + j]; };
@@ -274,14 +281,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     auto colkey = [&](int c, int excl) -> uint32_t {
         uint32_t k = 0u;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < TWO; ++t) {
             const int r = L + 64 * t;
             if (act[t] && r != excl) k = max(k, row_key(fabs(A(r, c)), r));
         }
         return group_max_u32<64>(k);
     };
     for (;;) {
-        const unsigned long long b0 = __ballot(act[0]), b1 = __ballot(act[1]);
+        const unsigned long long b0 = __ballot(act[0]), b1 = TWO == 2 ? __ballot(act[TWO - 1]) : 0ull;
         if (!b0 && !b1) break;
         const int c0 = b0 ? (int)__builtin_ctzll(b0) : 64 + (int)__builtin_ctzll(b1);
         // column c0 = row c0 by symmetry of the active block
@@ -295,7 +302,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             failed = true;  // zero column: singular
             break;
         }
-        for (int it = 0; it < 2 * kWideRows && need_rook; ++it) {
+        for (int it = 0; it < 2 * ROWS && need_rook; ++it) {
             const uint32_t rk = colkey(imax, imax);
             const int ri = rk ? 127 - (int)(rk & 127u) : imax;
             const double rowmax = rk ? fabs(A(ri, imax)) : 0.0;
@@ -303,7 +310,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             if (!(aii < alpha * rowmax)) {
                 kp = imax;
                 need_rook = false;
-            } else if (p == ri || rowmax <= colmax || it == 2 * kWideRows - 1) {
+            } else if (p == ri || rowmax <= colmax || it == 2 * ROWS - 1) {
                 kp = imax;
                 kstep = 2;
                 need_rook = false;
@@ -313,8 +320,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 imax = ri;
             }
         }
-        const int p0 = (kstep == 1) ? kp : p;
-        const int p1 = kp;  // equals p0 for a 1x1 pivot
+        // (the same on every lane; read from lane 0 so the compiler keeps the pivot
+        // bookkeeping and the column masks below in scalar registers)
+        kstep = __builtin_amdgcn_readfirstlane(kstep);
+        const int p0 = __builtin_amdgcn_readfirstlane((kstep == 1) ? kp : p);
+        const int p1 = __builtin_amdgcn_readfirstlane(kp);  // equals p0 for a 1x1 pivot
         const double d0 = A(p0, p0), d1 = A(p1, p1), bq = A(p0, p1);
         const double det = d0 * d1 - bq * bq;
         if ((kstep == 1 && d0 == 0.0) || (kstep == 2 && det == 0.0)) {
@@ -332,8 +342,26 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         }
         const double* U0 = s_A + (int64_t)p0 * lda;
         const double* U1 = s_A + (int64_t)p1 * lda;
+        // The columns of the still-active rows are updated eight at a time up to the next
+        // multiple of eight past the right-hand sides (padding columns, never read):
+        // uniform column offsets become immediate offsets of paired LDS loads, all loads
+        // of a chunk are issued before its stores, and per element the FMAs run in the
+        // same order. Columns of rows frozen earlier are never read again (the pivot
+        // search reads active columns, the back substitution reads a frozen row only at
+        // the columns active when it froze), so a chunk holding only such columns is
+        // skipped; the others are updated whole.
+        const int nc8 = (n + RC + 7) & ~7;
+        // a chunk of eight columns none of which is active any more (and no right-hand
+        // side) is skipped: bit j of (k0, k1) is set for the columns still needed
+        unsigned long long k0 = b0, k1 = b1;
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int c = 0; c < RC; ++c) {
+            const int j = n + c;
+            if (j < 64) k0 |= 1ull << j;
+            else k1 |= 1ull << (j - 64);
+        }
+#pragma unroll
+        for (int t = 0; t < TWO; ++t) {
             const int r = L + 64 * t;
             if (!act[t] || r == p0 || r == p1) continue;
             const double u0 = U0[r];
@@ -347,12 +375,36 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             }
             const double ml0 = -l0, ml1 = -l1;
             double* Ar = s_A + (int64_t)r * lda;
-            for (int j = 0; j < ncol; ++j) Ar[j] = fma(ml0, U0[j], Ar[j]);
-            if (kstep == 2)
-                for (int j = 0; j < ncol; ++j) Ar[j] = fma(ml1, U1[j], Ar[j]);
+            auto skip = [&](int j0) { return (((j0 < 64 ? k0 : k1) >> (j0 & 63)) & 0xffull) == 0ull; };
+            if (kstep == 1) {
+                for (int j0 = 0; j0 < nc8; j0 += 8) {
+                    if (skip(j0)) continue;
+                    double a[8], v0[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        a[q] = Ar[j0 + q];
+                        v0[q] = U0[j0 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) Ar[j0 + q] = fma(ml0, v0[q], a[q]);
+                }
+            } else {
+                for (int j0 = 0; j0 < nc8; j0 += 8) {
+                    if (skip(j0)) continue;
+                    double a[8], v0[8], v1[8];
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        a[q] = Ar[j0 + q];
+                        v0[q] = U0[j0 + q];
+                        v1[q] = U1[j0 + q];
+                    }
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) Ar[j0 + q] = fma(ml1, v1[q], fma(ml0, v0[q], a[q]));
+                }
+            }
         }
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < TWO; ++t) {
             const int r = L + 64 * t;
             if (act[t] && r == p0) {
                 act[t] = false;
@@ -369,9 +421,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     __syncthreads();
 
     // ---- back substitution over the frozen rows, last pivot first
-    double x[2][RC], acc[2][RC];
+    double x[TWO][RC], acc[TWO][RC];
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < TWO; ++t)
 #pragma unroll
         for (int c = 0; c < RC; ++c) {
             x[t][c] = 0.0;
@@ -380,12 +432,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     if (!failed) {
         const int nslots = slot;
         for (int s = nslots - 1; s >= 0; --s) {
-            const int blk = s_blk[s];
-            const int bprev = s > 0 ? s_blk[s - 1] : -1;
+            // (pivot bookkeeping read from LDS is the same on every lane: scalar registers)
+            const int blk = __builtin_amdgcn_readfirstlane(s_blk[s]);
+            const int bprev = __builtin_amdgcn_readfirstlane(s > 0 ? s_blk[s - 1] : -1);
             const bool one = blk == 1, two = (blk == 0) && (bprev == 2);
             if (!one && !two) continue;  // first slot of a 2x2 block: solved with the second
-            const int p = one ? s_piv[s] : s_piv[s - 1];
-            const int q = two ? s_piv[s] : p;
+            const int p = __builtin_amdgcn_readfirstlane(one ? s_piv[s] : s_piv[s - 1]);
+            const int q = __builtin_amdgcn_readfirstlane(two ? s_piv[s] : p);
             const int pl = p & 63, pt = p >> 6, ql = q & 63, qt = q >> 6;
             const double dp = A(p, p), dq = A(q, q), bq = A(p, q);
             const double idet = 1.0 / (dp * dq - bq * bq);
@@ -395,13 +448,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 if (c >= n_cols) continue;
                 // y - acc of rows p and q, from their owning lanes
                 const double rp0 = (L < n) ? A(L, n + c) - acc[0][c] : 0.0;
-                const double rp1 = (L + 64 < n) ? A(L + 64, n + c) - acc[1][c] : 0.0;
-                const double yp = pt ? __shfl(rp1, pl, 64) : __shfl(rp0, pl, 64);
-                const double yq = qt ? __shfl(rp1, ql, 64) : __shfl(rp0, ql, 64);
+                const double rp1 = (TWO == 2 && L + 64 < n) ? A(L + 64, n + c) - acc[TWO - 1][c] : 0.0;
+                const double yp = pt ? rdlane(rp1, pl) : rdlane(rp0, pl);
+                const double yq = qt ? rdlane(rp1, ql) : rdlane(rp0, ql);
                 const double x0 = two ? (yp * dq - yq * bq) * idet : yp * dinv;
                 const double x1 = two ? (yq * dp - yp * bq) * idet : 0.0;
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
+                for (int t = 0; t < TWO; ++t) {
                     const int r = L + 64 * t;
                     if (r == p) x[t][c] = x0;
                     if (two && r == q) x[t][c] = x1;
@@ -414,7 +467,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
 
     // ---- outputs: weights of the point rows, multipliers of the polynomial rows
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < TWO; ++t) {
         const int i = L + 64 * t;
         if (i >= n) continue;
 #pragma unroll
@@ -431,7 +484,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
 }
 
 static int rc_class(int n_cols) { return n_cols <= 1 ? 1 : (n_cols <= 4 ? 4 : 8); }
-static int wide_lda(int rows_cap, int n_cols) { return (rows_cap + rc_class(n_cols)) | 1; }
+// odd, and at least the columns and right-hand sides rounded up to a multiple of eight
+static int wide_lda(int rows_cap, int n_cols) { return ((rows_cap + rc_class(n_cols) + 7) & ~7) | 1; }
 
 // the system plus the kernel's static tables within 64 KB of LDS
 constexpr int64_t kWideStaticLds = 8 * 1024;
@@ -448,8 +502,12 @@ static hipError_t launch_wide_rc(const BuildDev& P, int rows_cap, int64_t n_work
     Q.n_blocks = n_work;
     const int lda = wide_lda(rows_cap, P.n_cols);
     const size_t lds = (size_t)rows_cap * lda * sizeof(double);
-    hipLaunchKernelGGL((k_build_wide<DIM, RC>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q, rows_cap,
-                       lda);
+    if (rows_cap <= 64)
+        hipLaunchKernelGGL((k_build_wide<DIM, RC, 1>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
+                           rows_cap, lda);
+    else
+        hipLaunchKernelGGL((k_build_wide<DIM, RC, 2>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
+                           rows_cap, lda);
     return hipGetLastError();
 }
 

@@ -19,11 +19,13 @@ __global__ __launch_bounds__(256) void k_bbox_blocks(const double* __restrict__ 
     __shared__ double smn[3][256], smx[3][256];
     double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
     for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
-        for (int d = 0; d < dim; ++d) {
-            const double v = x[i * dim + d];
-            mn[d] = fmin(mn[d], v);
-            mx[d] = fmax(mx[d], v);
-        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d)  // constant indexes keep mn/mx in registers
+            if (d < dim) {
+                const double v = x[i * dim + d];
+                mn[d] = fmin(mn[d], v);
+                mx[d] = fmax(mx[d], v);
+            }
     for (int d = 0; d < 3; ++d) {
         smn[d][threadIdx.x] = mn[d];
         smx[d][threadIdx.x] = mx[d];
@@ -62,11 +64,13 @@ __global__ __launch_bounds__(256) void k_morton(const double* __restrict__ x, in
     const double lim = (double)((1ull << bits) - 1ull);
     const double lo[3] = {lo0, lo1, lo2}, sc[3] = {sc0, sc1, sc2};
     uint64_t k = 0;
-    for (int d = 0; d < dim; ++d) {
-        double q = (x[i * dim + d] - lo[d]) * sc[d];
-        q = q < 0.0 ? 0.0 : (q > lim ? lim : q);
-        k |= spread_bits((uint64_t)q, bits, dim) << d;
-    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        if (d < dim) {
+            double q = (x[i * dim + d] - lo[d]) * sc[d];
+            q = q < 0.0 ? 0.0 : (q > lim ? lim : q);
+            k |= spread_bits((uint64_t)q, bits, dim) << d;
+        }
     key[i] = k;
     idx[i] = (int32_t)i;
 }
