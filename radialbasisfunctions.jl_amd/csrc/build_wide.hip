@@ -16,6 +16,19 @@ namespace rbfx {
 
 static_assert(RBFX_MAX_N <= 128, "at most two rows per lane");
 
+// maximum over the 64 lanes, as a scalar: rotations within each row of 16 lanes, then the
+// four row maxima read from lanes 0, 16, 32 and 48 (no LDS permute)
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t k)
+{
+    k = max(k, (uint32_t)dpp_i<0x121>((int)k));
+    k = max(k, (uint32_t)dpp_i<0x122>((int)k));
+    k = max(k, (uint32_t)dpp_i<0x124>((int)k));
+    k = max(k, (uint32_t)dpp_i<0x128>((int)k));
+    const uint32_t a = (uint32_t)__builtin_amdgcn_readlane((int)k, 0), b = (uint32_t)__builtin_amdgcn_readlane((int)k, 16);
+    const uint32_t c = (uint32_t)__builtin_amdgcn_readlane((int)k, 32), d = (uint32_t)__builtin_amdgcn_readlane((int)k, 48);
+    return max(max(a, b), max(c, d));
+}
+
 __device__ __forceinline__ uint32_t row_key(double v, int row)
 {
     const uint32_t hi = (uint32_t)((unsigned long long)__double_as_longlong(v) >> 32);
@@ -28,10 +41,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                                                                                   int lda)
 {
     constexpr int ROWS = 64 * TWO;
-    extern __shared__ __attribute__((aligned(16))) double s_A[];  // rows_cap x lda
-    __shared__ double s_xi[ROWS][DIM];
-    __shared__ double s_ba[ROWS], s_bb[ROWS], s_bn[ROWS][DIM];
-    __shared__ int s_bt[ROWS];
+    // the system (rows_cap x lda), then the point tables of the assembly, sized by rows_cap
+    // so that small systems leave LDS for more wavefronts per CU
+    extern __shared__ __attribute__((aligned(16))) double s_A[];
+    double(*s_xi)[DIM] = reinterpret_cast<double(*)[DIM]>(s_A + (int64_t)rows_cap * lda);
+    double(*s_bn)[DIM] = s_xi + rows_cap;
+    double* s_ba = reinterpret_cast<double*>(s_bn + rows_cap);
+    double* s_bb = s_ba + rows_cap;
+    int* s_bt = reinterpret_cast<int*>(s_bb + rows_cap);
     __shared__ int s_piv[ROWS], s_blk[ROWS];
     __shared__ int s_mono[kMaxMono][3];
     __shared__ TermDev s_terms[RBFX_MAX_TERMS];
@@ -277,15 +294,24 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     int slot = 0;
     bool failed = false;
     auto A = [&](int i, int j) -> double { return s_A[(int64_t)i * lda + j]; };
-    // largest |A(r, c)| over the active rows r != excl, as a packed (magnitude, row) key
+    // Column c of the active rows goes to this lane's registers (v[t] = A(L + 64 t, c)),
+    // and the largest |A(r, c)| over the active rows r != excl comes back as a packed
+    // (magnitude, row) key. The pivot tests then take single entries of that column from
+    // their owning lanes (colval) instead of another LDS round trip.
+    double v[TWO];
     auto colkey = [&](int c, int excl) -> uint32_t {
         uint32_t k = 0u;
 #pragma unroll
         for (int t = 0; t < TWO; ++t) {
             const int r = L + 64 * t;
-            if (act[t] && r != excl) k = max(k, row_key(fabs(A(r, c)), r));
+            v[t] = act[t] ? A(r, c) : 0.0;
+            if (act[t] && r != excl) k = max(k, row_key(fabs(v[t]), r));
         }
-        return group_max_u32<64>(k);
+        return wave_max_u32(k);
+    };
+    auto colval = [&](int r) -> double {
+        const int rr = __builtin_amdgcn_readfirstlane(r);
+        return (TWO == 2 && rr >= 64) ? rdlane(v[TWO - 1], rr & 63) : rdlane(v[0], rr & 63);
     };
     for (;;) {
         const unsigned long long b0 = __ballot(act[0]), b1 = TWO == 2 ? __ballot(act[TWO - 1]) : 0ull;
@@ -294,9 +320,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         // column c0 = row c0 by symmetry of the active block
         const uint32_t ck = colkey(c0, c0);
         int imax = ck ? 127 - (int)(ck & 127u) : c0;
-        double colmax = ck ? fabs(A(imax, c0)) : 0.0;
-        const double absakk = fabs(A(c0, c0));
+        const double akk = colval(c0);
+        double colmax = ck ? fabs(colval(imax)) : 0.0;
+        const double absakk = fabs(akk);
         int kstep = 1, p = c0, kp = c0;
+        // A(p, p), A(kp, kp) and, for a 2x2 pivot, A(p, kp) as the search ends
+        double app = akk, akp = akk, apk = akk;
         bool need_rook = absakk < alpha * colmax;
         if (!(absakk > 0.0 || colmax > 0.0)) {
             failed = true;  // zero column: singular
@@ -305,17 +334,22 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         for (int it = 0; it < 2 * ROWS && need_rook; ++it) {
             const uint32_t rk = colkey(imax, imax);
             const int ri = rk ? 127 - (int)(rk & 127u) : imax;
-            const double rowmax = rk ? fabs(A(ri, imax)) : 0.0;
-            const double aii = fabs(A(imax, imax));
+            const double rowmax = rk ? fabs(colval(ri)) : 0.0;
+            const double aim = colval(imax);
+            const double aii = fabs(aim);
             if (!(aii < alpha * rowmax)) {
                 kp = imax;
+                akp = aim;
                 need_rook = false;
             } else if (p == ri || rowmax <= colmax || it == 2 * ROWS - 1) {
                 kp = imax;
                 kstep = 2;
+                akp = aim;
+                apk = colval(p);  // column imax holds A(p, imax)
                 need_rook = false;
             } else {
                 p = imax;
+                app = aim;
                 colmax = rowmax;
                 imax = ri;
             }
@@ -325,7 +359,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         kstep = __builtin_amdgcn_readfirstlane(kstep);
         const int p0 = __builtin_amdgcn_readfirstlane((kstep == 1) ? kp : p);
         const int p1 = __builtin_amdgcn_readfirstlane(kp);  // equals p0 for a 1x1 pivot
-        const double d0 = A(p0, p0), d1 = A(p1, p1), bq = A(p0, p1);
+        // A(p0, p0), A(p1, p1), A(p0, p1)
+        const double d0 = kstep == 1 ? akp : app, d1 = akp, bq = kstep == 1 ? akp : apk;
         const double det = d0 * d1 - bq * bq;
         if ((kstep == 1 && d0 == 0.0) || (kstep == 2 && det == 0.0)) {
             failed = true;
@@ -486,13 +521,19 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
 static int rc_class(int n_cols) { return n_cols <= 1 ? 1 : (n_cols <= 4 ? 4 : 8); }
 // odd, and at least the columns and right-hand sides rounded up to a multiple of eight
 static int wide_lda(int rows_cap, int n_cols) { return ((rows_cap + rc_class(n_cols) + 7) & ~7) | 1; }
+// dynamic LDS: the system, then per row two coordinate triples (points, normals), two
+// boundary coefficients and a boundary type
+static size_t wide_lds(int rows_cap, int n_cols, int dim)
+{
+    return (size_t)rows_cap * wide_lda(rows_cap, n_cols) * 8 + (size_t)rows_cap * (2 * dim + 2) * 8 +
+           (size_t)rows_cap * 4;
+}
 
 // the system plus the kernel's static tables within 64 KB of LDS
-constexpr int64_t kWideStaticLds = 8 * 1024;
+constexpr int64_t kWideStaticLds = 2 * 1024;
 bool wide_fits(int rows_cap, int n_cols)
 {
-    return rows_cap >= 1 && rows_cap <= RBFX_MAX_N &&
-           (int64_t)rows_cap * wide_lda(rows_cap, n_cols) * 8 + kWideStaticLds <= 65536;
+    return rows_cap >= 1 && rows_cap <= RBFX_MAX_N && (int64_t)wide_lds(rows_cap, n_cols, 3) + kWideStaticLds <= 65536;
 }
 
 template <int DIM, int RC>
@@ -501,7 +542,7 @@ static hipError_t launch_wide_rc(const BuildDev& P, int rows_cap, int64_t n_work
     BuildDev Q = P;
     Q.n_blocks = n_work;
     const int lda = wide_lda(rows_cap, P.n_cols);
-    const size_t lds = (size_t)rows_cap * lda * sizeof(double);
+    const size_t lds = wide_lds(rows_cap, P.n_cols, DIM);
     if (rows_cap <= 64)
         hipLaunchKernelGGL((k_build_wide<DIM, RC, 1>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
                            rows_cap, lda);
