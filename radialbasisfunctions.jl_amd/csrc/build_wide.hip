@@ -1,6 +1,6 @@
 // Fused build for stencil systems of 65..RBFX_MAX_N rows (k up to 60 with a degree-3
 // polynomial gives 70): one wavefront per stencil, the augmented system [A | b] in LDS
-// (row-major, odd leading dimension lda >= n + RC rounded up to 8), lane li owning rows li
+// (row-major, odd leading dimension lda >= n + RC rounded up to 4), lane li owning rows li
 // and li + 64.
 //
 // The arithmetic is that of k_build (build_kernel.h): centred, radius-scaled
@@ -42,13 +42,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
 {
     constexpr int ROWS = 64 * TWO;
     // the system (rows_cap x lda), then the point tables of the assembly, sized by rows_cap
-    // so that small systems leave LDS for more wavefronts per CU
+    // so that small systems leave LDS for more wavefronts per CU: the scaled points, and
+    // for builds with boundary data the normals, boundary coefficients and types
     extern __shared__ __attribute__((aligned(16))) double s_A[];
     double(*s_xi)[DIM] = reinterpret_cast<double(*)[DIM]>(s_A + (int64_t)rows_cap * lda);
     double(*s_bn)[DIM] = s_xi + rows_cap;
     double* s_ba = reinterpret_cast<double*>(s_bn + rows_cap);
     double* s_bb = s_ba + rows_cap;
     int* s_bt = reinterpret_cast<int*>(s_bb + rows_cap);
+    const bool bnd_tables = P.is_boundary != nullptr;
     __shared__ int s_piv[ROWS], s_blk[ROWS];
     __shared__ int s_mono[kMaxMono][3];
     __shared__ TermDev s_terms[RBFX_MAX_TERMS];
@@ -145,9 +147,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 s_xi[q][d] = D[t][d] / h;
                 eq = eq && (D[t][d] == 0.0);
             }
-            s_bt[q] = bt;
-            s_ba[q] = ba;
-            s_bb[q] = bb / h;  // beta acts through a first derivative
+            if (bnd_tables) {
+                s_bt[q] = bt;
+                s_ba[q] = ba;
+                s_bb[q] = bb / h;  // beta acts through a first derivative
+            }
         }
         at_eval[t] = __ballot(eq);
     }
@@ -183,9 +187,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 xi[d] = s_xi[pi][d];
                 bn[d] = hermite ? s_bn[pi][d] : 0.0;
             }
-            bt = s_bt[pi];
-            ba = s_ba[pi];
-            bb = s_bb[pi];
+            if (hermite) {
+                bt = s_bt[pi];
+                ba = s_ba[pi];
+                bb = s_bb[pi];
+            }
         }
         for (int j = 0; j < n; ++j) {
             double v = 0.0;
@@ -200,7 +206,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 double xj[DIM];
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) xj[d] = s_xi[jj][d];
-                const int tj = s_bt[jj];
+                const int tj = hermite ? s_bt[jj] : 0;
                 if (is_poly) {
                     // polynomial row i: monomial i at point jj
                     v = mono_op<DIM>(s_mono[i], RBFX_OP_VALUE, 0, 0, xj);
@@ -377,15 +383,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         }
         const double* U0 = s_A + (int64_t)p0 * lda;
         const double* U1 = s_A + (int64_t)p1 * lda;
-        // The columns of the still-active rows are updated eight at a time up to the next
-        // multiple of eight past the right-hand sides (padding columns, never read):
+        // The columns of the still-active rows are updated eight (the last four) at a time
+        // up to the next multiple of four past the right-hand sides (padding, never read):
         // uniform column offsets become immediate offsets of paired LDS loads, all loads
         // of a chunk are issued before its stores, and per element the FMAs run in the
         // same order. Columns of rows frozen earlier are never read again (the pivot
         // search reads active columns, the back substitution reads a frozen row only at
         // the columns active when it froze), so a chunk holding only such columns is
         // skipped; the others are updated whole.
-        const int nc8 = (n + RC + 7) & ~7;
+        const int nc4 = (n + RC + 3) & ~3;
         // a chunk of eight columns none of which is active any more (and no right-hand
         // side) is skipped: bit j of (k0, k1) is set for the columns still needed
         unsigned long long k0 = b0, k1 = b1;
@@ -410,32 +416,43 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             }
             const double ml0 = -l0, ml1 = -l1;
             double* Ar = s_A + (int64_t)r * lda;
-            auto skip = [&](int j0) { return (((j0 < 64 ? k0 : k1) >> (j0 & 63)) & 0xffull) == 0ull; };
+            auto skip = [&](int j0) { return (((j0 < 64 ? k0 : k1) >> (j0 & 63)) & 0xffull) == 0ull; };  // (8 columns)
+            // chunks of eight, then one of four where the columns end there; a 1x1 and a
+            // 2x2 pivot each have their own loop, so a chunk's loads stay together
+            auto chunk1 = [&](auto W, int j0) __attribute__((always_inline)) {
+                constexpr int CW = decltype(W)::value;
+                double a[CW], v0[CW];
+#pragma unroll
+                for (int q = 0; q < CW; ++q) {
+                    a[q] = Ar[j0 + q];
+                    v0[q] = U0[j0 + q];
+                }
+#pragma unroll
+                for (int q = 0; q < CW; ++q) Ar[j0 + q] = fma(ml0, v0[q], a[q]);
+            };
+            auto chunk2 = [&](auto W, int j0) __attribute__((always_inline)) {
+                constexpr int CW = decltype(W)::value;
+                double a[CW], v0[CW], v1[CW];
+#pragma unroll
+                for (int q = 0; q < CW; ++q) {
+                    a[q] = Ar[j0 + q];
+                    v0[q] = U0[j0 + q];
+                    v1[q] = U1[j0 + q];
+                }
+#pragma unroll
+                for (int q = 0; q < CW; ++q) Ar[j0 + q] = fma(ml1, v1[q], fma(ml0, v0[q], a[q]));
+            };
+            using C8 = std::integral_constant<int, 8>;
+            using C4 = std::integral_constant<int, 4>;
+            int j0 = 0;
             if (kstep == 1) {
-                for (int j0 = 0; j0 < nc8; j0 += 8) {
-                    if (skip(j0)) continue;
-                    double a[8], v0[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        a[q] = Ar[j0 + q];
-                        v0[q] = U0[j0 + q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) Ar[j0 + q] = fma(ml0, v0[q], a[q]);
-                }
+                for (; j0 + 8 <= nc4; j0 += 8)
+                    if (!skip(j0)) chunk1(C8{}, j0);
+                if (j0 < nc4 && !skip(j0)) chunk1(C4{}, j0);
             } else {
-                for (int j0 = 0; j0 < nc8; j0 += 8) {
-                    if (skip(j0)) continue;
-                    double a[8], v0[8], v1[8];
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) {
-                        a[q] = Ar[j0 + q];
-                        v0[q] = U0[j0 + q];
-                        v1[q] = U1[j0 + q];
-                    }
-#pragma unroll
-                    for (int q = 0; q < 8; ++q) Ar[j0 + q] = fma(ml1, v1[q], fma(ml0, v0[q], a[q]));
-                }
+                for (; j0 + 8 <= nc4; j0 += 8)
+                    if (!skip(j0)) chunk2(C8{}, j0);
+                if (j0 < nc4 && !skip(j0)) chunk2(C4{}, j0);
             }
         }
 #pragma unroll
@@ -519,21 +536,22 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
 }
 
 static int rc_class(int n_cols) { return n_cols <= 1 ? 1 : (n_cols <= 4 ? 4 : 8); }
-// odd, and at least the columns and right-hand sides rounded up to a multiple of eight
-static int wide_lda(int rows_cap, int n_cols) { return ((rows_cap + rc_class(n_cols) + 7) & ~7) | 1; }
-// dynamic LDS: the system, then per row two coordinate triples (points, normals), two
-// boundary coefficients and a boundary type
-static size_t wide_lds(int rows_cap, int n_cols, int dim)
+// odd, and at least the columns and right-hand sides rounded up to a multiple of four
+static int wide_lda(int rows_cap, int n_cols) { return ((rows_cap + rc_class(n_cols) + 3) & ~3) | 1; }
+// dynamic LDS: the system, then per row the scaled point and, for builds with boundary
+// data, the normal, two boundary coefficients and a boundary type
+static size_t wide_lds(int rows_cap, int n_cols, int dim, bool bnd)
 {
-    return (size_t)rows_cap * wide_lda(rows_cap, n_cols) * 8 + (size_t)rows_cap * (2 * dim + 2) * 8 +
-           (size_t)rows_cap * 4;
+    return (size_t)rows_cap * wide_lda(rows_cap, n_cols) * 8 + (size_t)rows_cap * dim * 8 +
+           (bnd ? (size_t)rows_cap * (dim + 2) * 8 + (size_t)rows_cap * 4 : 0);
 }
 
 // the system plus the kernel's static tables within 64 KB of LDS
 constexpr int64_t kWideStaticLds = 2 * 1024;
 bool wide_fits(int rows_cap, int n_cols)
 {
-    return rows_cap >= 1 && rows_cap <= RBFX_MAX_N && (int64_t)wide_lds(rows_cap, n_cols, 3) + kWideStaticLds <= 65536;
+    return rows_cap >= 1 && rows_cap <= RBFX_MAX_N &&
+           (int64_t)wide_lds(rows_cap, n_cols, 3, true) + kWideStaticLds <= 65536;
 }
 
 template <int DIM, int RC>
@@ -542,7 +560,7 @@ static hipError_t launch_wide_rc(const BuildDev& P, int rows_cap, int64_t n_work
     BuildDev Q = P;
     Q.n_blocks = n_work;
     const int lda = wide_lda(rows_cap, P.n_cols);
-    const size_t lds = wide_lds(rows_cap, P.n_cols, DIM);
+    const size_t lds = wide_lds(rows_cap, P.n_cols, DIM, P.is_boundary != nullptr);
     if (rows_cap <= 64)
         hipLaunchKernelGGL((k_build_wide<DIM, RC, 1>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
                            rows_cap, lda);
