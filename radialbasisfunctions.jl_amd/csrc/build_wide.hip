@@ -483,24 +483,46 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         }
     if (!failed) {
         const int nslots = slot;
+        // the right-hand sides (final once every row has frozen) and the pivot list go to
+        // registers: each slot then costs one round of LDS loads (the pivot block and the
+        // lane's entries in the pivot columns), issued together
+        double y[TWO][RC];
+        int pv[TWO], bk[TWO];
+#pragma unroll
+        for (int t = 0; t < TWO; ++t) {
+            const int r = L + 64 * t;
+#pragma unroll
+            for (int c = 0; c < RC; ++c) y[t][c] = r < n ? A(r, n + c) : 0.0;
+            pv[t] = r < nslots ? s_piv[r] : 0;
+            bk[t] = r < nslots ? s_blk[r] : -1;
+        }
+        auto slot_of = [&](const int(&v)[TWO], int sl) -> int {
+            return (TWO == 2 && sl >= 64) ? rdlane(v[TWO - 1], sl & 63) : rdlane(v[0], sl & 63);
+        };
         for (int s = nslots - 1; s >= 0; --s) {
-            // (pivot bookkeeping read from LDS is the same on every lane: scalar registers)
-            const int blk = __builtin_amdgcn_readfirstlane(s_blk[s]);
-            const int bprev = __builtin_amdgcn_readfirstlane(s > 0 ? s_blk[s - 1] : -1);
+            const int blk = slot_of(bk, s);
+            const int bprev = s > 0 ? slot_of(bk, s - 1) : -1;
             const bool one = blk == 1, two = (blk == 0) && (bprev == 2);
             if (!one && !two) continue;  // first slot of a 2x2 block: solved with the second
-            const int p = __builtin_amdgcn_readfirstlane(one ? s_piv[s] : s_piv[s - 1]);
-            const int q = __builtin_amdgcn_readfirstlane(two ? s_piv[s] : p);
+            const int p = slot_of(pv, one ? s : s - 1);
+            const int q = two ? slot_of(pv, s) : p;
             const int pl = p & 63, pt = p >> 6, ql = q & 63, qt = q >> 6;
             const double dp = A(p, p), dq = A(q, q), bq = A(p, q);
+            double arp[TWO], arq[TWO];
+#pragma unroll
+            for (int t = 0; t < TWO; ++t) {
+                const int r = L + 64 * t;
+                arp[t] = r < n ? A(r, p) : 0.0;
+                arq[t] = (two && r < n) ? A(r, q) : 0.0;
+            }
             const double idet = 1.0 / (dp * dq - bq * bq);
             const double dinv = 1.0 / dp;
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
                 if (c >= n_cols) continue;
                 // y - acc of rows p and q, from their owning lanes
-                const double rp0 = (L < n) ? A(L, n + c) - acc[0][c] : 0.0;
-                const double rp1 = (TWO == 2 && L + 64 < n) ? A(L + 64, n + c) - acc[TWO - 1][c] : 0.0;
+                const double rp0 = y[0][c] - acc[0][c];
+                const double rp1 = TWO == 2 ? y[TWO - 1][c] - acc[TWO - 1][c] : 0.0;
                 const double yp = pt ? rdlane(rp1, pl) : rdlane(rp0, pl);
                 const double yq = qt ? rdlane(rp1, ql) : rdlane(rp0, ql);
                 const double x0 = two ? (yp * dq - yq * bq) * idet : yp * dinv;
@@ -511,7 +533,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                     if (r == p) x[t][c] = x0;
                     if (two && r == q) x[t][c] = x1;
                     const bool upd = r < n && (one ? (myslot[t] < s) : (myslot[t] < s - 1));
-                    if (upd) acc[t][c] += two ? A(r, p) * x0 + A(r, q) * x1 : A(r, p) * x0;
+                    if (upd) acc[t][c] += two ? arp[t] * x0 + arq[t] * x1 : arp[t] * x0;
                 }
             }
         }
