@@ -193,51 +193,62 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 bb = s_bb[pi];
             }
         }
-        for (int j = 0; j < n; ++j) {
-            double v = 0.0;
-            if (j < M) {
-                // point row: monomial j at the own point (Neumann/Robin rows: alpha P + beta dP/dn)
-                if (!is_poly) {
-                    v = mono_op<DIM>(s_mono[j], RBFX_OP_VALUE, 0, 0, xi);
-                    if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[j], bn, xi);
-                }
-            } else {
-                const int jj = j - M;
-                double xj[DIM];
-#pragma unroll
-                for (int d = 0; d < DIM; ++d) xj[d] = s_xi[jj][d];
-                const int tj = hermite ? s_bt[jj] : 0;
-                if (is_poly) {
-                    // polynomial row i: monomial i at point jj
-                    v = mono_op<DIM>(s_mono[i], RBFX_OP_VALUE, 0, 0, xj);
-                    if (hermite && tj == 2) {
-                        double nj[DIM];
-#pragma unroll
-                        for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
-                        v = s_ba[jj] * v + s_bb[jj] * mono_normal<DIM>(s_mono[i], nj, xj);
+        // (the row loop is instantiated once per basis kind: no kind switch per entry)
+        auto assemble_row = [&](auto KC) __attribute__((always_inline)) {
+            for (int j = 0; j < n; ++j) {
+                double v = 0.0;
+                if (j < M) {
+                    // point row: monomial j at the own point (Neumann/Robin rows: alpha P + beta dP/dn)
+                    if (!is_poly) {
+                        v = mono_op<DIM>(s_mono[j], RBFX_OP_VALUE, 0, 0, xi);
+                        if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[j], bn, xi);
                     }
                 } else {
-                    double Dj[DIM], q2 = 0.0;
+                    const int jj = j - M;
+                    double xj[DIM];
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) {
-                        Dj[d] = xi[d] - xj[d];
-                        q2 += Dj[d] * Dj[d];
-                    }
-                    v = phi_r2(B, q2);
-                    if (hermite && (bt == 2 || tj == 2)) {
-                        double nj[DIM];
+                    for (int d = 0; d < DIM; ++d) xj[d] = s_xi[jj][d];
+                    const int tj = hermite ? s_bt[jj] : 0;
+                    if (is_poly) {
+                        // polynomial row i: monomial i at point jj
+                        v = mono_op<DIM>(s_mono[i], RBFX_OP_VALUE, 0, 0, xj);
+                        if (hermite && tj == 2) {
+                            double nj[DIM];
 #pragma unroll
-                        for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
-                        double Di, Djn, D2;
-                        phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
-                        const double aj = s_ba[jj], bj = s_bb[jj];
-                        if (bt != 2) v = aj * v - bj * Djn;
-                        else if (tj != 2) v = ba * v + bb * Di;
-                        else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
+                            for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
+                            v = s_ba[jj] * v + s_bb[jj] * mono_normal<DIM>(s_mono[i], nj, xj);
+                        }
+                    } else {
+                        double Dj[DIM], q2 = 0.0;
+#pragma unroll
+                        for (int d = 0; d < DIM; ++d) {
+                            Dj[d] = xi[d] - xj[d];
+                            q2 += Dj[d] * Dj[d];
+                        }
+                        v = phi_r2_k<decltype(KC)::value>(B.eps, q2);
+                        if (hermite && (bt == 2 || tj == 2)) {
+                            double nj[DIM];
+#pragma unroll
+                            for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
+                            double Di, Djn, D2;
+                            phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
+                            const double aj = s_ba[jj], bj = s_bb[jj];
+                            if (bt != 2) v = aj * v - bj * Djn;
+                            else if (tj != 2) v = ba * v + bb * Di;
+                            else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
+                        }
                     }
                 }
+                Ai[j] = v;
             }
-            Ai[j] = v;
+        };
+        switch (B.kind) {
+        case RBFX_PHS1: assemble_row(std::integral_constant<int, RBFX_PHS1>{}); break;
+        case RBFX_PHS3: assemble_row(std::integral_constant<int, RBFX_PHS3>{}); break;
+        case RBFX_PHS5: assemble_row(std::integral_constant<int, RBFX_PHS5>{}); break;
+        case RBFX_PHS7: assemble_row(std::integral_constant<int, RBFX_PHS7>{}); break;
+        case RBFX_IMQ: assemble_row(std::integral_constant<int, RBFX_IMQ>{}); break;
+        default: assemble_row(std::integral_constant<int, RBFX_GAUSSIAN>{}); break;
         }
         // right-hand sides
         double bcol[RC];

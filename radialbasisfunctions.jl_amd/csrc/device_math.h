@@ -43,6 +43,19 @@ __device__ __forceinline__ double phi_r2(const BasisDev& b, double r2)
     }
 }
 
+// phi_r2 with the kind a compile-time constant (same arithmetic), for loops instantiated
+// once per kind
+template <int KIND>
+__device__ __forceinline__ double phi_r2_k(double eps, double r2)
+{
+    if constexpr (KIND == RBFX_PHS1) return sqrt(r2);
+    else if constexpr (KIND == RBFX_PHS3) return r2 * sqrt(r2);
+    else if constexpr (KIND == RBFX_PHS5) return (r2 * r2) * sqrt(r2);
+    else if constexpr (KIND == RBFX_PHS7) return (r2 * r2 * r2) * sqrt(r2);
+    else if constexpr (KIND == RBFX_IMQ) return 1.0 / sqrt((eps * eps) * r2 + 1.0);
+    else return exp(-(eps * eps) * r2);
+}
+
 // One operator term applied to phi at (x, xi), D = x - xi, r2 = |D|^2.
 template <int DIM>
 __device__ __forceinline__ double phi_term(const BasisDev& b, int kind, int d1, int d2,
