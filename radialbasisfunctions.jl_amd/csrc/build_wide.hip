@@ -193,62 +193,70 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 bb = s_bb[pi];
             }
         }
-        // (the row loop is instantiated once per basis kind: no kind switch per entry)
-        auto assemble_row = [&](auto KC) __attribute__((always_inline)) {
-            for (int j = 0; j < n; ++j) {
-                double v = 0.0;
-                if (j < M) {
-                    // point row: monomial j at the own point (Neumann/Robin rows: alpha P + beta dP/dn)
-                    if (!is_poly) {
-                        v = mono_op<DIM>(s_mono[j], RBFX_OP_VALUE, 0, 0, xi);
-                        if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[j], bn, xi);
+        // columns 0..M-1. A point row holds its monomials; the polynomial rows' entries at
+        // the point columns are the same numbers (symmetry of the saddle-point matrix), so
+        // each point lane also writes its value into column i of polynomial row a, and the
+        // polynomial lanes only zero their corner block.
+        for (int a = 0; a < M; ++a) {
+            if (is_poly) {
+                Ai[a] = 0.0;
+            } else {
+                double v = mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi);
+                if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[a], bn, xi);
+                Ai[a] = v;
+                s_A[(int64_t)a * lda + i] = v;
+            }
+        }
+        // columns M..n-1 of the point rows: phi between the points (instantiated once per
+        // basis kind; without boundary data four independent entries per iteration)
+        auto point_cols = [&](auto KC) __attribute__((always_inline)) {
+            if (is_poly) return;
+            if (!hermite) {
+#pragma unroll 4
+                for (int jj = 0; jj < kk; ++jj) {
+                    double q2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) {
+                        const double t_ = xi[d] - s_xi[jj][d];
+                        q2 += t_ * t_;
                     }
-                } else {
-                    const int jj = j - M;
-                    double xj[DIM];
-#pragma unroll
-                    for (int d = 0; d < DIM; ++d) xj[d] = s_xi[jj][d];
-                    const int tj = hermite ? s_bt[jj] : 0;
-                    if (is_poly) {
-                        // polynomial row i: monomial i at point jj
-                        v = mono_op<DIM>(s_mono[i], RBFX_OP_VALUE, 0, 0, xj);
-                        if (hermite && tj == 2) {
-                            double nj[DIM];
-#pragma unroll
-                            for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
-                            v = s_ba[jj] * v + s_bb[jj] * mono_normal<DIM>(s_mono[i], nj, xj);
-                        }
-                    } else {
-                        double Dj[DIM], q2 = 0.0;
-#pragma unroll
-                        for (int d = 0; d < DIM; ++d) {
-                            Dj[d] = xi[d] - xj[d];
-                            q2 += Dj[d] * Dj[d];
-                        }
-                        v = phi_r2_k<decltype(KC)::value>(B.eps, q2);
-                        if (hermite && (bt == 2 || tj == 2)) {
-                            double nj[DIM];
-#pragma unroll
-                            for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
-                            double Di, Djn, D2;
-                            phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
-                            const double aj = s_ba[jj], bj = s_bb[jj];
-                            if (bt != 2) v = aj * v - bj * Djn;
-                            else if (tj != 2) v = ba * v + bb * Di;
-                            else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
-                        }
-                    }
+                    Ai[M + jj] = phi_r2_k<decltype(KC)::value>(B.eps, q2);
                 }
-                Ai[j] = v;
+                return;
+            }
+            for (int jj = 0; jj < kk; ++jj) {
+                double xj[DIM];
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) xj[d] = s_xi[jj][d];
+                const int tj = s_bt[jj];
+                double Dj[DIM], q2 = 0.0;
+#pragma unroll
+                for (int d = 0; d < DIM; ++d) {
+                    Dj[d] = xi[d] - xj[d];
+                    q2 += Dj[d] * Dj[d];
+                }
+                double v = phi_r2_k<decltype(KC)::value>(B.eps, q2);
+                if (bt == 2 || tj == 2) {
+                    double nj[DIM];
+#pragma unroll
+                    for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
+                    double Di, Djn, D2;
+                    phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
+                    const double aj = s_ba[jj], bj = s_bb[jj];
+                    if (bt != 2) v = aj * v - bj * Djn;
+                    else if (tj != 2) v = ba * v + bb * Di;
+                    else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
+                }
+                Ai[M + jj] = v;
             }
         };
         switch (B.kind) {
-        case RBFX_PHS1: assemble_row(std::integral_constant<int, RBFX_PHS1>{}); break;
-        case RBFX_PHS3: assemble_row(std::integral_constant<int, RBFX_PHS3>{}); break;
-        case RBFX_PHS5: assemble_row(std::integral_constant<int, RBFX_PHS5>{}); break;
-        case RBFX_PHS7: assemble_row(std::integral_constant<int, RBFX_PHS7>{}); break;
-        case RBFX_IMQ: assemble_row(std::integral_constant<int, RBFX_IMQ>{}); break;
-        default: assemble_row(std::integral_constant<int, RBFX_GAUSSIAN>{}); break;
+        case RBFX_PHS1: point_cols(std::integral_constant<int, RBFX_PHS1>{}); break;
+        case RBFX_PHS3: point_cols(std::integral_constant<int, RBFX_PHS3>{}); break;
+        case RBFX_PHS5: point_cols(std::integral_constant<int, RBFX_PHS5>{}); break;
+        case RBFX_PHS7: point_cols(std::integral_constant<int, RBFX_PHS7>{}); break;
+        case RBFX_IMQ: point_cols(std::integral_constant<int, RBFX_IMQ>{}); break;
+        default: point_cols(std::integral_constant<int, RBFX_GAUSSIAN>{}); break;
         }
         // right-hand sides
         double bcol[RC];
@@ -526,8 +534,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 arp[t] = r < n ? A(r, p) : 0.0;
                 arq[t] = (two && r < n) ? A(r, q) : 0.0;
             }
-            const double idet = 1.0 / (dp * dq - bq * bq);
-            const double dinv = 1.0 / dp;
+            // (one division per pivot: a 1x1 pivot needs only 1/dp, a 2x2 only 1/det)
+            double idet = 0.0, dinv = 0.0;
+            if (two) idet = 1.0 / (dp * dq - bq * bq);
+            else dinv = 1.0 / dp;
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
                 if (c >= n_cols) continue;
