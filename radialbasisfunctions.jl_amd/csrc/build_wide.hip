@@ -1,7 +1,7 @@
 // Fused build for stencil systems of 65..RBFX_MAX_N rows (k up to 60 with a degree-3
 // polynomial gives 70): one wavefront per stencil, the augmented system [A | b] in LDS
-// (row-major, odd leading dimension lda >= n + RC rounded up to 4), lane li owning rows li
-// and li + 64.
+// (row-major: the n columns, padding to a multiple of eight, then the RC right-hand sides;
+// odd leading dimension), lane li owning rows li and li + 64.
 //
 // The arithmetic is that of k_build (build_kernel.h): centred, radius-scaled
 // saddle-point system assembled in FP64, rook-pivoted LDL^T (pivot tests of Ashcraft,
@@ -9,7 +9,8 @@
 // still-active rows, and a back substitution over the rows frozen at their pivot step.
 // Only the storage differs: the rows live in LDS instead of registers, so a pivot-row
 // read is an LDS load and the pivot search reads the column entries of the lane's own
-// rows. The trailing update runs eight columns per LDS round trip.
+// rows. The active columns are kept in a prefix of every row, and the trailing update
+// runs over that prefix eight columns per LDS round trip.
 #include "build_kernel.h"
 
 namespace rbfx {
@@ -80,6 +81,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         return;
     }
     const int n = M + kk;
+    const int rhs0 = (n + 7) & ~7;  // position of the first right-hand side in a row
 
     // ---- the stencil points this lane loads: q = L (and q = L + 64)
     double xe[DIM];
@@ -303,7 +305,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             }
         }
 #pragma unroll
-        for (int c = 0; c < RC; ++c) Ai[n + c] = bcol[c];
+        for (int c = 0; c < RC; ++c) Ai[rhs0 + c] = bcol[c];
     }
     __syncthreads();
 
@@ -323,13 +325,28 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     // and the largest |A(r, c)| over the active rows r != excl comes back as a packed
     // (magnitude, row) key. The pivot tests then take single entries of that column from
     // their owning lanes (colval) instead of another LDS round trip.
+    //
+    // Column positions: the columns of the active rows sit in positions 0..n_act-1 of
+    // every row (the right-hand sides at rhs0.., past the padding). Freezing a pivot swaps
+    // its column with the last active position in every row, frozen rows included, so
+    // the trailing update only touches the active prefix, and an entry (r, c) is always
+    // at position pos(c) of row r. pos[t] is the position of column L + 64 t.
+    int pos[TWO];
+#pragma unroll
+    for (int t = 0; t < TWO; ++t) pos[t] = L + 64 * t;
+    int n_act = n;
+    auto posof = [&](int c) -> int {
+        const int cc = __builtin_amdgcn_readfirstlane(c);
+        return (TWO == 2 && cc >= 64) ? rdlane(pos[TWO - 1], cc & 63) : rdlane(pos[0], cc & 63);
+    };
     double v[TWO];
     auto colkey = [&](int c, int excl) -> uint32_t {
         uint32_t k = 0u;
+        const int pc = posof(c);
 #pragma unroll
         for (int t = 0; t < TWO; ++t) {
             const int r = L + 64 * t;
-            v[t] = act[t] ? A(r, c) : 0.0;
+            v[t] = act[t] ? A(r, pc) : 0.0;
             if (act[t] && r != excl) k = max(k, row_key(fabs(v[t]), r));
         }
         return wave_max_u32(k);
@@ -400,78 +417,81 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 s_blk[slot + 1] = 0;
             }
         }
+        // freeze: the pivot columns move to the end of the active prefix in every row
+        auto swap_to = [&](int c, int last) {
+            const int pc = posof(c);
+            if (pc == last) return;
+#pragma unroll
+            for (int t = 0; t < TWO; ++t) {
+                const int r = L + 64 * t;
+                if (r < n) {
+                    double* Ar = s_A + (int64_t)r * lda;
+                    const double x0 = Ar[pc], x1 = Ar[last];
+                    Ar[pc] = x1;
+                    Ar[last] = x0;
+                }
+                if (pos[t] == last) pos[t] = pc;
+                else if (r == c) pos[t] = last;
+            }
+        };
+        swap_to(p0, n_act - 1);
+        if (kstep == 2) swap_to(p1, n_act - 2);
+        n_act -= kstep;
         const double* U0 = s_A + (int64_t)p0 * lda;
         const double* U1 = s_A + (int64_t)p1 * lda;
-        // The columns of the still-active rows are updated eight (the last four) at a time
-        // up to the next multiple of four past the right-hand sides (padding, never read):
-        // uniform column offsets become immediate offsets of paired LDS loads, all loads
-        // of a chunk are issued before its stores, and per element the FMAs run in the
-        // same order. Columns of rows frozen earlier are never read again (the pivot
-        // search reads active columns, the back substitution reads a frozen row only at
-        // the columns active when it froze), so a chunk holding only such columns is
-        // skipped; the others are updated whole.
-        const int nc4 = (n + RC + 3) & ~3;
-        // a chunk of eight columns none of which is active any more (and no right-hand
-        // side) is skipped: bit j of (k0, k1) is set for the columns still needed
-        unsigned long long k0 = b0, k1 = b1;
-#pragma unroll
-        for (int c = 0; c < RC; ++c) {
-            const int j = n + c;
-            if (j < 64) k0 |= 1ull << j;
-            else k1 |= 1ull << (j - 64);
-        }
+        // The active prefix (rounded up to eight: the padding before rhs0 and the frozen
+        // columns past the prefix are never read) and the right-hand sides are updated
+        // eight columns at a time: uniform column offsets become immediate offsets of
+        // paired LDS loads, all loads of a chunk are issued before its stores, and per
+        // element the FMAs run in the same order as a column-by-column update.
+        const int na8 = (n_act + 7) & ~7;
 #pragma unroll
         for (int t = 0; t < TWO; ++t) {
             const int r = L + 64 * t;
             if (!act[t] || r == p0 || r == p1) continue;
-            const double u0 = U0[r];
+            const double u0 = U0[pos[t]];
             double l0, l1 = 0.0;
             if (kstep == 1) {
                 l0 = u0 / d0;
             } else {
-                const double u1 = U1[r];
+                const double u1 = U1[pos[t]];
                 l0 = (u0 * d1 - u1 * bq) * idet;
                 l1 = (u1 * d0 - u0 * bq) * idet;
             }
             const double ml0 = -l0, ml1 = -l1;
             double* Ar = s_A + (int64_t)r * lda;
-            auto skip = [&](int j0) { return (((j0 < 64 ? k0 : k1) >> (j0 & 63)) & 0xffull) == 0ull; };  // (8 columns)
-            // chunks of eight, then one of four where the columns end there; a 1x1 and a
-            // 2x2 pivot each have their own loop, so a chunk's loads stay together
+            // a 1x1 and a 2x2 pivot each have their own loop, so a chunk's loads stay together
             auto chunk1 = [&](auto W, int j0) __attribute__((always_inline)) {
                 constexpr int CW = decltype(W)::value;
-                double a[CW], v0[CW];
+                double a[CW], w0[CW];
 #pragma unroll
                 for (int q = 0; q < CW; ++q) {
                     a[q] = Ar[j0 + q];
-                    v0[q] = U0[j0 + q];
+                    w0[q] = U0[j0 + q];
                 }
 #pragma unroll
-                for (int q = 0; q < CW; ++q) Ar[j0 + q] = fma(ml0, v0[q], a[q]);
+                for (int q = 0; q < CW; ++q) Ar[j0 + q] = fma(ml0, w0[q], a[q]);
             };
             auto chunk2 = [&](auto W, int j0) __attribute__((always_inline)) {
                 constexpr int CW = decltype(W)::value;
-                double a[CW], v0[CW], v1[CW];
+                double a[CW], w0[CW], w1[CW];
 #pragma unroll
                 for (int q = 0; q < CW; ++q) {
                     a[q] = Ar[j0 + q];
-                    v0[q] = U0[j0 + q];
-                    v1[q] = U1[j0 + q];
+                    w0[q] = U0[j0 + q];
+                    w1[q] = U1[j0 + q];
                 }
 #pragma unroll
-                for (int q = 0; q < CW; ++q) Ar[j0 + q] = fma(ml1, v1[q], fma(ml0, v0[q], a[q]));
+                for (int q = 0; q < CW; ++q) Ar[j0 + q] = fma(ml1, w1[q], fma(ml0, w0[q], a[q]));
             };
             using C8 = std::integral_constant<int, 8>;
-            using C4 = std::integral_constant<int, 4>;
-            int j0 = 0;
+            using CR = std::integral_constant<int, RC>;
             if (kstep == 1) {
-                for (; j0 + 8 <= nc4; j0 += 8)
-                    if (!skip(j0)) chunk1(C8{}, j0);
-                if (j0 < nc4 && !skip(j0)) chunk1(C4{}, j0);
+                for (int j0 = 0; j0 < na8; j0 += 8) chunk1(C8{}, j0);
+                chunk1(CR{}, rhs0);
             } else {
-                for (; j0 + 8 <= nc4; j0 += 8)
-                    if (!skip(j0)) chunk2(C8{}, j0);
-                if (j0 < nc4 && !skip(j0)) chunk2(C4{}, j0);
+                for (int j0 = 0; j0 < na8; j0 += 8) chunk2(C8{}, j0);
+                chunk2(CR{}, rhs0);
             }
         }
 #pragma unroll
@@ -511,7 +531,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         for (int t = 0; t < TWO; ++t) {
             const int r = L + 64 * t;
 #pragma unroll
-            for (int c = 0; c < RC; ++c) y[t][c] = r < n ? A(r, n + c) : 0.0;
+            for (int c = 0; c < RC; ++c) y[t][c] = r < n ? A(r, rhs0 + c) : 0.0;
             pv[t] = r < nslots ? s_piv[r] : 0;
             bk[t] = r < nslots ? s_blk[r] : -1;
         }
@@ -526,13 +546,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             const int p = slot_of(pv, one ? s : s - 1);
             const int q = two ? slot_of(pv, s) : p;
             const int pl = p & 63, pt = p >> 6, ql = q & 63, qt = q >> 6;
-            const double dp = A(p, p), dq = A(q, q), bq = A(p, q);
+            const int pp = posof(p), pq = posof(q);
+            const double dp = A(p, pp), dq = A(q, pq), bq = A(p, pq);
             double arp[TWO], arq[TWO];
 #pragma unroll
             for (int t = 0; t < TWO; ++t) {
                 const int r = L + 64 * t;
-                arp[t] = r < n ? A(r, p) : 0.0;
-                arq[t] = (two && r < n) ? A(r, q) : 0.0;
+                arp[t] = r < n ? A(r, pp) : 0.0;
+                arq[t] = (two && r < n) ? A(r, pq) : 0.0;
             }
             // (one division per pivot: a 1x1 pivot needs only 1/dp, a 2x2 only 1/det)
             double idet = 0.0, dinv = 0.0;
@@ -579,8 +600,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
 }
 
 static int rc_class(int n_cols) { return n_cols <= 1 ? 1 : (n_cols <= 4 ? 4 : 8); }
-// odd, and at least the columns and right-hand sides rounded up to a multiple of four
-static int wide_lda(int rows_cap, int n_cols) { return ((rows_cap + rc_class(n_cols) + 3) & ~3) | 1; }
+// odd: the columns rounded up to a multiple of eight, then the right-hand sides
+static int wide_lda(int rows_cap, int n_cols) { return (((rows_cap + 7) & ~7) + rc_class(n_cols)) | 1; }
 // dynamic LDS: the system, then per row the scaled point and, for builds with boundary
 // data, the normal, two boundary coefficients and a boundary type
 static size_t wide_lds(int rows_cap, int n_cols, int dim, bool bnd)
