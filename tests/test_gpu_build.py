@@ -1076,6 +1076,66 @@ def test_degree_map_validation(RBF, O, gpu):
         assert np.quantile(err, 0.99) <= 1e-9, (dg, np.quantile(err, 0.99))
 
 
+@pytest.mark.parametrize("kernel", [-1, 2])
+def test_c5_full_density_subbox(RBF, O, gpu, kernel):
+    """The regrid at the full configuration's density and shape parameter: a 0.1-wide box
+    of the 4M-source cloud (4,000 sources, IMQ eps = 3 absolute, so eps * h is the full
+    configuration's) and 1,728 jittered targets of its 1/199 grid. Stencil condition
+    numbers reach 4e18 (median 5e15), the flat regime where FP64 weights carry errors of
+    order kappa u. Per stencil the weights stay within max(1e-10, kappa_2 u) of the
+    extended-precision oracle; applied to 16 smooth fields they stay within 10x the
+    CPU FP64 restatement's distance from that oracle (measured 3.4e-4 relative), and the
+    interpolation error is the oracle's. kernel: -1 automatic, 2 the LDS kernel pinned."""
+    import torch
+
+    rng = np.random.default_rng(77)
+    side = 0.1
+    src = rng.random((int(4_000_000 * side ** 3), 3)) * side
+    hg = 1.0 / 199
+    g = np.arange(0.02, 0.08, hg)
+    tg = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    tg = tg + rng.uniform(-0.3 * hg, 0.3 * hg, tg.shape)
+    if kernel >= 0:
+        for b in (1, 2):
+            RBF._lib.tune_bucket(b, kernel)
+    try:
+        op = RBF.regrid(src, tg, basis=RBF.IMQ(3.0, poly_deg=1), k=30, device=gpu)
+        if kernel >= 0:
+            assert RBF._lib.last_build_buckets()[1][1] == kernel
+    finally:
+        for b in (1, 2):
+            RBF._lib.tune_bucket(b, -1)
+    adj = op.adj_idx.cpu().numpy()
+    basis = O.basis_struct(O.abi.RBFX_IMQ, 1, 3.0)
+    value = _op(O, 1, [(0, O.abi.OP_VALUE, 0, 0, 1.0)])
+    wg = op.vals.cpu().numpy()
+    check_against_truth(O, wg, src, tg, adj, basis, value, 30)
+    _, _, w64, _, rc = O.build(src, tg, adj, basis, value, scaled=True)
+    _, _, wld, _, rc2 = O.build(src, tg, adj, basis, value, scaled=True, long_double=True)
+    assert rc == 0 and rc2 == 0
+    kv = rng.integers(-6, 7, (16, 8, 3))
+    ph = rng.uniform(0, 2 * np.pi, (16, 8))
+    amp = rng.standard_normal((16, 8))
+
+    def fields(x):
+        return np.stack([(amp[f] * np.cos(2 * np.pi * (x @ kv[f].T) + ph[f])).sum(1) for f in range(16)], 1)
+
+    Fs, Ft = fields(src), fields(tg)
+    Y = op(torch.as_tensor(Fs, device=gpu)).cpu().numpy()
+
+    def applied(w):
+        return np.einsum("ek,ekf->ef", w[0].reshape(-1, 30), Fs[adj])
+
+    yld = applied(wld)
+    scale = np.abs(yld).max(0)
+    e64 = (np.abs(applied(w64) - yld) / scale).max()
+    eg = (np.abs(Y - yld) / scale).max()
+    assert eg <= max(1e-9, 10 * e64), (eg, e64)
+    nrmse = np.sqrt(np.mean((Y - Ft) ** 2)) / np.sqrt(np.mean(Ft ** 2))
+    nrmse_ld = np.sqrt(np.mean((yld - Ft) ** 2)) / np.sqrt(np.mean(Ft ** 2))
+    assert nrmse <= 1.05 * nrmse_ld + 1e-12 and nrmse < 1e-3, (nrmse, nrmse_ld)
+
+
 def test_c5_shape_regrid_imq_multifield(RBF, O, gpu):
     """3-D regrid: uniform sources to a jittered target grid, IMQ eps=3, degree 1,
     k=30, FP32 build + FP64 refinement, 16 fields applied at once; the applied values
