@@ -1,7 +1,10 @@
 // Fused build for stencil systems of 65..RBFX_MAX_N rows (k up to 60 with a degree-3
 // polynomial gives 70): one wavefront per stencil, the augmented system [A | b] in LDS
 // (row-major: the n columns, padding to a multiple of eight, then the RC right-hand sides;
-// odd leading dimension), lane li owning rows li and li + 64.
+// odd leading dimension). Lane li owns row m0 + li, and for systems of more than 64 rows
+// (m0 = n - 64 > 0) also row li < m0: the second rows are the first ones, the polynomial
+// rows, which the pivot order freezes within the first few steps, so the second pass of
+// the trailing update is skipped for the rest of the factorisation.
 //
 // The arithmetic is that of k_build (build_kernel.h): centred, radius-scaled
 // saddle-point system assembled in FP64, rook-pivoted LDL^T (pivot tests of Ashcraft,
@@ -82,6 +85,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     }
     const int n = M + kk;
     const int rhs0 = (n + 7) & ~7;  // position of the first right-hand side in a row
+    // row ownership: slot 0 of lane L holds row m0 + L, slot 1 (TWO == 2) row L < m0
+    const int m0 = TWO == 2 ? max(n - 64, 0) : 0;
+    auto rowof = [&](int t) -> int { return (TWO == 2 && t == 1) ? L : L + m0; };
+    auto owns = [&](int t) -> bool { return (TWO == 2 && t == 1) ? L < m0 : L + m0 < n; };
+    // the owner of row r (uniform): its lane and slot
+    auto lane_of = [&](int r) -> int { return r < m0 ? r : r - m0; };
+    auto slot_t = [&](int r) -> int { return (TWO == 2 && r < m0) ? 1 : 0; };
 
     // ---- the stencil points this lane loads: q = L (and q = L + 64)
     double xe[DIM];
@@ -172,11 +182,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     B.kind = P.basis_kind;
     B.eps = (B.kind == RBFX_IMQ || B.kind == RBFX_GAUSSIAN) ? P.eps * h : P.eps;
 
-    // ---- assembly (FP64): rows i = L (and L + 64); columns 0..n-1, then the RHS at n..n+RC-1
+    // ---- assembly (FP64): the lane's rows; columns 0..n-1, then the RHS at rhs0..rhs0+RC-1
 #pragma unroll
     for (int t = 0; t < TWO; ++t) {
-        const int i = L + 64 * t;
-        if (i >= n) continue;
+        const int i = rowof(t);
+        if (!owns(t)) continue;
         double* Ai = s_A + (int64_t)i * lda;
         const bool is_poly = i < M;
         const int pi = i - M;
@@ -315,13 +325,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     int myslot[TWO];
 #pragma unroll
     for (int t = 0; t < TWO; ++t) {
-        act[t] = L + 64 * t < n;
+        act[t] = owns(t);
         myslot[t] = 1 << 20;
     }
     int slot = 0;
     bool failed = false;
     auto A = [&](int i, int j) -> double { return s_A[(int64_t)i * lda + j]; };
-    // Column c of the active rows goes to this lane's registers (v[t] = A(L + 64 t, c)),
+    // Column c of the active rows goes to this lane's registers (v[t] = A(rowof(t), c)),
     // and the largest |A(r, c)| over the active rows r != excl comes back as a packed
     // (magnitude, row) key. The pivot tests then take single entries of that column from
     // their owning lanes (colval) instead of another LDS round trip.
@@ -330,14 +340,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     // every row (the right-hand sides at rhs0.., past the padding). Freezing a pivot swaps
     // its column with the last active position in every row, frozen rows included, so
     // the trailing update only touches the active prefix, and an entry (r, c) is always
-    // at position pos(c) of row r. pos[t] is the position of column L + 64 t.
+    // at position pos(c) of row r. pos[t] is the position of column rowof(t).
     int pos[TWO];
 #pragma unroll
-    for (int t = 0; t < TWO; ++t) pos[t] = L + 64 * t;
+    for (int t = 0; t < TWO; ++t) pos[t] = owns(t) ? rowof(t) : 1 << 20;
     int n_act = n;
     auto posof = [&](int c) -> int {
         const int cc = __builtin_amdgcn_readfirstlane(c);
-        return (TWO == 2 && cc >= 64) ? rdlane(pos[TWO - 1], cc & 63) : rdlane(pos[0], cc & 63);
+        return slot_t(cc) ? rdlane(pos[TWO - 1], lane_of(cc)) : rdlane(pos[0], lane_of(cc));
     };
     double v[TWO];
     auto colkey = [&](int c, int excl) -> uint32_t {
@@ -345,7 +355,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         const int pc = posof(c);
 #pragma unroll
         for (int t = 0; t < TWO; ++t) {
-            const int r = L + 64 * t;
+            const int r = rowof(t);
             v[t] = act[t] ? A(r, pc) : 0.0;
             if (act[t] && r != excl) k = max(k, row_key(fabs(v[t]), r));
         }
@@ -353,12 +363,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     };
     auto colval = [&](int r) -> double {
         const int rr = __builtin_amdgcn_readfirstlane(r);
-        return (TWO == 2 && rr >= 64) ? rdlane(v[TWO - 1], rr & 63) : rdlane(v[0], rr & 63);
+        return slot_t(rr) ? rdlane(v[TWO - 1], lane_of(rr)) : rdlane(v[0], lane_of(rr));
     };
     for (;;) {
         const unsigned long long b0 = __ballot(act[0]), b1 = TWO == 2 ? __ballot(act[TWO - 1]) : 0ull;
         if (!b0 && !b1) break;
-        const int c0 = b0 ? (int)__builtin_ctzll(b0) : 64 + (int)__builtin_ctzll(b1);
+        // the lowest active row: slot 1 holds rows 0..m0-1, slot 0 rows m0..n-1
+        const int c0 = b1 ? (int)__builtin_ctzll(b1) : m0 + (int)__builtin_ctzll(b0);
         // column c0 = row c0 by symmetry of the active block
         const uint32_t ck = colkey(c0, c0);
         int imax = ck ? 127 - (int)(ck & 127u) : c0;
@@ -423,13 +434,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             if (pc == last) return;
 #pragma unroll
             for (int t = 0; t < TWO; ++t) {
-                const int r = L + 64 * t;
-                if (r < n) {
-                    double* Ar = s_A + (int64_t)r * lda;
-                    const double x0 = Ar[pc], x1 = Ar[last];
-                    Ar[pc] = x1;
-                    Ar[last] = x0;
-                }
+                if (!owns(t)) continue;
+                const int r = rowof(t);
+                double* Ar = s_A + (int64_t)r * lda;
+                const double x0 = Ar[pc], x1 = Ar[last];
+                Ar[pc] = x1;
+                Ar[last] = x0;
                 if (pos[t] == last) pos[t] = pc;
                 else if (r == c) pos[t] = last;
             }
@@ -447,7 +457,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         const int na8 = (n_act + 7) & ~7;
 #pragma unroll
         for (int t = 0; t < TWO; ++t) {
-            const int r = L + 64 * t;
+            const int r = rowof(t);
             if (!act[t] || r == p0 || r == p1) continue;
             const double u0 = U0[pos[t]];
             double l0, l1 = 0.0;
@@ -496,7 +506,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         }
 #pragma unroll
         for (int t = 0; t < TWO; ++t) {
-            const int r = L + 64 * t;
+            const int r = rowof(t);
             if (act[t] && r == p0) {
                 act[t] = false;
                 myslot[t] = slot;
@@ -529,11 +539,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
         int pv[TWO], bk[TWO];
 #pragma unroll
         for (int t = 0; t < TWO; ++t) {
-            const int r = L + 64 * t;
 #pragma unroll
-            for (int c = 0; c < RC; ++c) y[t][c] = r < n ? A(r, rhs0 + c) : 0.0;
-            pv[t] = r < nslots ? s_piv[r] : 0;
-            bk[t] = r < nslots ? s_blk[r] : -1;
+            for (int c = 0; c < RC; ++c) y[t][c] = owns(t) ? A(rowof(t), rhs0 + c) : 0.0;
+            // slot sl of the pivot list sits in lane sl % 64, register sl / 64
+            const int sl = L + 64 * t;
+            pv[t] = sl < nslots ? s_piv[sl] : 0;
+            bk[t] = sl < nslots ? s_blk[sl] : -1;
         }
         auto slot_of = [&](const int(&v)[TWO], int sl) -> int {
             return (TWO == 2 && sl >= 64) ? rdlane(v[TWO - 1], sl & 63) : rdlane(v[0], sl & 63);
@@ -545,15 +556,15 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             if (!one && !two) continue;  // first slot of a 2x2 block: solved with the second
             const int p = slot_of(pv, one ? s : s - 1);
             const int q = two ? slot_of(pv, s) : p;
-            const int pl = p & 63, pt = p >> 6, ql = q & 63, qt = q >> 6;
+            const int pl = lane_of(p), pt = slot_t(p), ql = lane_of(q), qt = slot_t(q);
             const int pp = posof(p), pq = posof(q);
             const double dp = A(p, pp), dq = A(q, pq), bq = A(p, pq);
             double arp[TWO], arq[TWO];
 #pragma unroll
             for (int t = 0; t < TWO; ++t) {
-                const int r = L + 64 * t;
-                arp[t] = r < n ? A(r, pp) : 0.0;
-                arq[t] = (two && r < n) ? A(r, pq) : 0.0;
+                const int r = rowof(t);
+                arp[t] = owns(t) ? A(r, pp) : 0.0;
+                arq[t] = (two && owns(t)) ? A(r, pq) : 0.0;
             }
             // (one division per pivot: a 1x1 pivot needs only 1/dp, a 2x2 only 1/det)
             double idet = 0.0, dinv = 0.0;
@@ -571,10 +582,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
                 const double x1 = two ? (yq * dp - yp * bq) * idet : 0.0;
 #pragma unroll
                 for (int t = 0; t < TWO; ++t) {
-                    const int r = L + 64 * t;
+                    const int r = rowof(t);
+                    if (!owns(t)) continue;
                     if (r == p) x[t][c] = x0;
                     if (two && r == q) x[t][c] = x1;
-                    const bool upd = r < n && (one ? (myslot[t] < s) : (myslot[t] < s - 1));
+                    const bool upd = one ? (myslot[t] < s) : (myslot[t] < s - 1);
                     if (upd) acc[t][c] += two ? arp[t] * x0 + arq[t] * x1 : arp[t] * x0;
                 }
             }
@@ -584,8 +596,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
     // ---- outputs: weights of the point rows, multipliers of the polynomial rows
 #pragma unroll
     for (int t = 0; t < TWO; ++t) {
-        const int i = L + 64 * t;
-        if (i >= n) continue;
+        const int i = rowof(t);
+        if (!owns(t)) continue;
 #pragma unroll
         for (int c = 0; c < RC; ++c) {
             if (c >= n_cols) continue;
@@ -593,7 +605,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_wid
             if (i >= M) P.out_val[(int64_t)c * P.out_nnz + o0 + (i - M)] = v;
             else if (P.out_lambda) P.out_lambda[((int64_t)c * P.n_eval + e) * P.n_mono + i] = v;
         }
-        // row i holds stencil point i - M (the points were loaded by lane q % 64)
+        // row i holds stencil point i - M
         if (i >= M && P.out_col) P.out_col[o0 + (i - M)] = P.adj_idx[s0 + (i - M)];
     }
     if (L == 0 && P.status) P.status[e] = failed ? RBFX_ST_SINGULAR : P.ok_status;
