@@ -149,7 +149,7 @@ def time_apply(R, op, X, stream, reps=20):
 
 # ------------------------------------------------------------------ other configurations
 KERNEL_NAMES = {-1: "-", 0: "regs x2", 1: "regs", 2: "lds", 3: "regs x2 (fp64)", 4: "regs (fp64)",
-                5: "null-space cholesky", 6: "null-space cholesky 8x3"}
+                5: "null-space cholesky", 6: "null-space cholesky 8x3", 7: "lds packed"}
 # solve kernels of the 4M 2-D Laplacian (k = 20, m = 6) by kernel id:
 # (description, name in a rocprofv3 trace, what bounds it)
 SOLVE_KERNELS = {

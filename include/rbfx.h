@@ -344,7 +344,8 @@ int rbfx_last_build_timing(double ms[4]);
 /* Size buckets of the last build on this thread (no reference counterpart): stencils
  * with n = k + m <= 32, 33..64 and 65..RBFX_MAX_N, and the kernel each ran on
  * (0: registers, two stencils per wavefront; 1: registers, one; 2: system in LDS,
- * FP64; 3, 4: kernels 0, 1 in FP64 within an FP32 + refinement build; -1: none). Uneven builds (ragged k or a degree map) classify stencils into these
+ * FP64; 3, 4: kernels 0, 1 in FP64 within an FP32 + refinement build; 5, 6: null-space
+ * Cholesky; 7: packed lower triangle in LDS, FP64; -1: none). Uneven builds (ragged k or a degree map) classify stencils into these
  * buckets; the kernel of a bucket is timed on a sample the first time a
  * configuration is seen, and the fastest is kept (counts of -1: not read back).  */
 int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3]);
@@ -357,9 +358,11 @@ int rbfx_last_build_buckets(int64_t n[3], int32_t kernel[3]);
  * an FP32 + refinement build; 5 null-space Cholesky with one FP64 refinement step
  * (interior stencils of bases definite on the polynomial null space: PHS1 with degree
  * >= 0, PHS3 >= 1, PHS5 >= 2, PHS7 >= 3, IMQ, Gaussian; the stencils whose pivots or
- * refinement fail are re-solved by kernel 0 or 2); 6 the same with 8 lanes of three
- * rows per stencil (k <= 20, at most 6 monomials). A pinned kernel that cannot serve a build is
- * ignored by that build.                                                        */
+ * refinement fail are re-solved by kernel 0 or 7); 6 the same with 8 lanes of three
+ * rows per stencil (k <= 20, at most 6 monomials); 7 rook LDL^T with the packed lower
+ * triangle of the system in LDS (symmetric pivot moves; half the LDS of kernel 2, so
+ * more wavefronts per CU). A pinned kernel that cannot serve a build is ignored by that
+ * build.                                                                        */
 int rbfx_tune_bucket(int32_t bucket, int32_t kernel);
 
 /* Counters of the last rbfx_build_weights on this thread (no reference counterpart):

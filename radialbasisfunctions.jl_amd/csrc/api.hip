@@ -261,9 +261,11 @@ static int spmv_impl(const rbfx_spmv_args* a, bool async)
 //             build
 //   K_NSC, K_NSC2  null-space Cholesky (no pivot search) with one FP64 refinement
 //             step, for interior stencils of bases definite on the polynomial null
-//             space; the stencils it hands back are re-solved by K_REG32 / K_WIDE.
+//             space; the stencils it hands back are re-solved by K_REG32 / K_SYM.
 //             K_NSC2 maps 8 lanes of three rows to a stencil (eight stencils per
 //             wavefront, k <= 20, m <= 6).
+//   K_SYM     rook LDL^T with the packed lower triangle in LDS (n <= RBFX_MAX_N; half the
+//             LDS of K_WIDE, so more wavefronts per CU)
 enum {
     K_REG32 = 0,
     K_REG64 = 1,
@@ -272,14 +274,15 @@ enum {
     K_REG64_F64 = 4,
     K_NSC = 5,
     K_NSC2 = 6,
-    kNumKernels = 7
+    K_SYM = 7,
+    kNumKernels = 8
 };
 int g_force_kernel[3] = {-1, -1, -1};  // rbfx_tune_bucket overrides
 thread_local int64_t g_bucket_n[3];
 thread_local int32_t g_bucket_kernel[3];
 thread_local int64_t g_stats[8];       // rbfx_last_build_stats
 constexpr int64_t kTuneMin = 4096;     // stencils in a bucket before its mapping is timed
-constexpr int32_t kTuneSample = 16384;
+constexpr int32_t kTuneSample = 65536;
 
 // How a build runs: blocking (sizes its buffers, reads back statuses), asynchronous on
 // the caller's stream (capturable; needs buffers an earlier blocking build sized), or
@@ -309,6 +312,7 @@ static bool kernel_serves(int kid, int bucket, bool fp32, const Shape& sh, bool 
     // (in FP32 mode the 3-D kernel with 8 columns still spills at one wave: not served)
     case K_REG64: return bucket <= 1 && !(fp32 && sh.dim == 3 && sh.n_cols > 4);
     case K_WIDE: return bucket == 2 || (bucket == 1 && rbfx::wide_fits(64, sh.n_cols));
+    case K_SYM: return rbfx::sym_fits(bucket == 0 ? 32 : (bucket == 1 ? 64 : RBFX_MAX_N), sh.n_cols);
     case K_REG32_F64: return fp32 && bucket == 0;
     case K_REG64_F64: return fp32 && bucket <= 1;
     case K_NSC:
@@ -326,7 +330,7 @@ static bool kernel_serves(int kid, int bucket, bool fp32, const Shape& sh, bool 
 static int rook_for(int bucket, int n_cols)
 {
     if (bucket == 0) return K_REG32;
-    return rbfx::wide_fits(bucket == 1 ? 64 : RBFX_MAX_N, n_cols) ? K_WIDE : K_REG64;
+    return rbfx::sym_fits(bucket == 1 ? 64 : RBFX_MAX_N, n_cols) ? K_SYM : K_REG64;
 }
 
 // Where a null-space launch queues the stencils it hands back (n_eval entries) and
@@ -340,7 +344,8 @@ static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape
                                    bool fp32, int rows_cap, int64_t n_work, bool async, hipStream_t st)
 {
     rbfx::BuildDev Q = P;
-    if (fp32 && (kid == K_WIDE || kid == K_REG32_F64 || kid == K_REG64_F64 || is_nsc(kid))) Q.ok_status = RBFX_ST_FP64;
+    if (fp32 && (kid == K_WIDE || kid == K_SYM || kid == K_REG32_F64 || kid == K_REG64_F64 || is_nsc(kid)))
+        Q.ok_status = RBFX_ST_FP64;
     if (is_nsc(kid)) {
         // the null-space launch, then the rook kernel on the stencils it handed back
         const int cap = bucket == 0 ? 32 : (bucket == 1 ? 64 : RBFX_MAX_N);
@@ -365,6 +370,7 @@ static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape
         return launch_kernel_id(rook_for(bucket, Q.n_cols), F, sh, hb, bucket, false, rows_cap, nf, async, st);
     }
     if (kid == K_WIDE) return rbfx::launch_build_wide(Q, sh.dim, rows_cap, n_work, st);
+    if (kid == K_SYM) return rbfx::launch_build_sym(Q, sh.dim, rows_cap, n_work, st);
     const bool f32 = fp32 && (kid == K_REG32 || kid == K_REG64);
     return rbfx::launch_build(Q, sh.dim, (kid == K_REG32 || kid == K_REG32_F64) ? 32 : 64, f32, n_work, st);
 }
@@ -380,8 +386,8 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const 
                        Mode mode, hipStream_t st, int* kid)
 {
     const bool async = mode != Mode::kBlocking;
-    int dflt = bucket == 0 ? K_REG32 : K_WIDE;
-    if (bucket == 1 && !rbfx::wide_fits(64, sh.n_cols)) dflt = K_REG64;
+    int dflt = bucket == 0 ? K_REG32 : K_SYM;
+    if (bucket == 1 && !rbfx::sym_fits(64, sh.n_cols)) dflt = K_REG64;
     if (kernel_serves(K_NSC, bucket, fp32, sh, false)) dflt = K_NSC;
     if (kernel_serves(K_NSC2, bucket, fp32, sh, false)) dflt = K_NSC2;  // (the faster mapping where it fits)
     *kid = dflt;

@@ -98,10 +98,13 @@ __device__ __forceinline__ int stencil_n_mono(const BuildDev& P, int64_t e, int 
     return (d >= -1 && d <= P.poly_deg) ? n_mono_of(dim, d) : -1;  // -1: invalid degree
 }
 
-// FP64 build of stencils with n = k + m in 65..RBFX_MAX_N rows (system in LDS);
+// FP64 builds of stencils of up to RBFX_MAX_N rows with the system in LDS: the full
+// matrix (k_build_wide) or its packed lower triangle (k_build_sym, half the LDS);
 // rows_cap is the largest n in the launch
 hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n_work, hipStream_t s);
 bool wide_fits(int rows_cap, int n_cols);
+hipError_t launch_build_sym(const BuildDev& P, int dim, int rows_cap, int64_t n_work, hipStream_t s);
+bool sym_fits(int rows_cap, int n_cols);
 
 hipError_t launch_build(const BuildDev& P, int dim, int nmax, bool fp32, int64_t n_work, hipStream_t s);
 // the offsets and column indices of every work position of P (its list, or order, or
