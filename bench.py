@@ -348,7 +348,9 @@ def config_c3(R, dev):
         out[prec] = {"build_ms": ms, "stencils_per_s": n / (ms * 1e-3), "tflops": tf,
                      "frac_fp64_peak": tf / FP64_PEAK_TFLOPS, "status": _status(R, op),
                      "buckets_n<=32,33..64,65..": nbk,
-                     "bucket_kernels": [KERNEL_NAMES[k] for k in kern]}
+                     "bucket_kernels": [KERNEL_NAMES[k] for k in kern],
+                     "launch_plan": {1: "one launch per size bucket", 2: "one launch for all stencils"}.get(
+                         R._lib.last_build_stats()[2], "-")}
         if prec == "fp32_refined":
             x, y = d.unbind(1)
             f = torch.sin(4 * x) * torch.cos(3 * y)

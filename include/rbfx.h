@@ -367,9 +367,20 @@ int rbfx_tune_bucket(int32_t bucket, int32_t kernel);
 
 /* Counters of the last rbfx_build_weights on this thread (no reference counterpart):
  * [0] stencils the null-space kernels handed to the rook kernels (breakdown, refinement
- * not settling, Hermite or out-of-shape stencils), [1] autotuner sample launches, the
- * rest 0.                                                                        */
+ * not settling, Hermite or out-of-shape stencils), [1] autotuner sample launches, [2]
+ * the launch plan of an uneven build (1 one launch per size bucket, 2 one launch for
+ * all stencils; 0 for a build of one size), the rest 0.                          */
 int rbfx_last_build_stats(int64_t stats[8]);
+
+/* Pin the launch plan of uneven builds (ragged k or a degree map) for later builds in
+ * this process: 0 automatic, 1 one launch per size bucket after a classify pass (each
+ * bucket on its own kernel), 2 one launch of kernel 7 over every stencil in processing
+ * order, with no classify pass. The automatic choice is made the first time a
+ * configuration is timed (every nonempty bucket holding 4096+ stencils) by timing
+ * kernel 7, sized for the largest stencil, on each bucket's sample against that bucket's
+ * own choice; it is kept in the tuning map (bucket field 3, kernel field the plan).
+ * No reference counterpart (the north_star's one launch for stencils of uneven size). */
+int rbfx_tune_plan(int32_t plan);
 
 /* Save / load the autotuner's choices (no reference counterpart; the checkpoint of
  * the tuning map): one text line per configuration (dimension, operator columns,

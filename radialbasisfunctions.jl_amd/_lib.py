@@ -68,6 +68,8 @@ def lib():
                 L.rbfx_spatial_order.argtypes = [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p]
                 L.rbfx_last_build_stats.restype = C.c_int
                 L.rbfx_last_build_stats.argtypes = [C.POINTER(C.c_int64)]
+                L.rbfx_tune_plan.restype = C.c_int
+                L.rbfx_tune_plan.argtypes = [C.c_int32]
                 L.rbfx_tune_save.restype = C.c_int
                 L.rbfx_tune_save.argtypes = [C.c_char_p]
                 L.rbfx_tune_load.restype = C.c_int
@@ -101,7 +103,7 @@ def exported_symbols():
             "rbfx_stencil_features", "rbfx_spatial_order", "rbfx_build_weights_async", "rbfx_spmv_async",
             "rbfx_build_weights_noalloc", "rbfx_source_tiles",
             "rbfx_permute_rows", "rbfx_last_build_buckets", "rbfx_tune_bucket", "rbfx_last_build_stats",
-            "rbfx_tune_save", "rbfx_tune_load", "rbfx_tune_reset"]
+            "rbfx_tune_save", "rbfx_tune_load", "rbfx_tune_reset", "rbfx_tune_plan"]
 
 
 def _check(rc):
@@ -178,16 +180,23 @@ def tune_bucket(bucket, kernel):
     """Pin the kernel of a size bucket (-1: automatic). Kernel ids as rbfx_tune_bucket
     documents them: 0 rook registers x2, 1 rook registers (pinned only), 2 rook LDS,
     3/4 those in FP64 within FP32 mode, 5 null-space Cholesky, 6 the same, 8 lanes of
-    three rows per stencil."""
+    three rows per stencil, 7 rook LDS with the packed lower triangle."""
     _check(lib().rbfx_tune_bucket(int(bucket), int(kernel)))
 
 
 def last_build_stats():
     """[stencils the null-space kernels handed to the rook kernels, autotuner sample
-    launches, 0, ...] of the last build on this thread."""
+    launches, launch plan of an uneven build (1 per bucket, 2 one launch; 0 one size),
+    0, ...] of the last build on this thread."""
     st = (C.c_int64 * 8)()
     _check(lib().rbfx_last_build_stats(st))
     return list(st)
+
+
+def tune_plan(plan):
+    """Pin the launch plan of uneven builds: 0 automatic, 1 one launch per size bucket,
+    2 one launch of the packed LDS kernel over all stencils."""
+    _check(lib().rbfx_tune_plan(int(plan)))
 
 
 def tune_save(path):

@@ -278,6 +278,12 @@ enum {
     kNumKernels = 8
 };
 int g_force_kernel[3] = {-1, -1, -1};  // rbfx_tune_bucket overrides
+// Launch plans of an uneven build: one launch per size bucket, each on its own kernel,
+// after a classify pass (kPlanBucketed), or one launch of the packed LDS kernel over
+// every stencil in processing order, no classify pass (kPlanSingle). The autotuner keeps
+// its choice in the tuning map under the pseudo-bucket kPlanKey.
+constexpr int kPlanBucketed = 1, kPlanSingle = 2, kPlanKey = 3;
+int g_force_plan = 0;  // rbfx_tune_plan override (0: automatic)
 thread_local int64_t g_bucket_n[3];
 thread_local int32_t g_bucket_kernel[3];
 thread_local int64_t g_stats[8];       // rbfx_last_build_stats
@@ -383,8 +389,9 @@ static hipError_t launch_kernel_id(int kid, const rbfx::BuildDev& P, const Shape
 // builds), or the first `sample` stencils in processing order.
 static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const Shape& sh, DeviceState& ds,
                        const HandBack& hb, int bucket, bool fp32, int rows_cap, int64_t count, int32_t* sample_len,
-                       Mode mode, hipStream_t st, int* kid)
+                       Mode mode, hipStream_t st, int* kid, float* timed_per_stencil = nullptr)
 {
+    if (timed_per_stencil) *timed_per_stencil = -1.f;  // (set only when this call times a sample)
     const bool async = mode != Mode::kBlocking;
     int dflt = bucket == 0 ? K_REG32 : K_SYM;
     if (bucket == 1 && !rbfx::sym_fits(64, sh.n_cols)) dflt = K_REG64;
@@ -459,6 +466,23 @@ static int pick_kernel(const rbfx_build_args* a, const rbfx::BuildDev& Q, const 
     }
     g_stats[0] = 0;  // the sample runs' hand-backs are not the build's
     ds.tuned[key] = *kid;
+    if (timed_per_stencil) *timed_per_stencil = best / (float)sample;
+    return RBFX_OK;
+}
+
+// Device time of one launch of kernel kid on a sample (a warm-up run, then a timed one).
+static int time_sample(int kid, const rbfx::BuildDev& S, const Shape& sh, DeviceState& ds, const HandBack& hb,
+                       int bucket, bool fp32, int rows_cap, int32_t sample, hipStream_t st, float* ms)
+{
+    for (int i = 0; i < 2; ++i)
+        if (!ds.tev[i]) HIPCHK(hipEventCreate(&ds.tev[i]));
+    HIPCHK(launch_kernel_id(kid, S, sh, hb, bucket, fp32, rows_cap, sample, false, st));
+    HIPCHK(hipEventRecord(ds.tev[0], st));
+    HIPCHK(launch_kernel_id(kid, S, sh, hb, bucket, fp32, rows_cap, sample, false, st));
+    HIPCHK(hipEventRecord(ds.tev[1], st));
+    HIPCHK(hipEventSynchronize(ds.tev[1]));
+    g_stats[1] += 2;
+    HIPCHK(hipEventElapsedTime(ms, ds.tev[0], ds.tev[1]));
     return RBFX_OK;
 }
 
@@ -541,6 +565,32 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Sh
     }
     if (mode == Mode::kNoAlloc)  // (the entry point refuses these builds before any work)
         return set_err(RBFX_EUNSUPPORTED, "a build without allocation needs one size class (k + m <= 32 or fixed k)");
+    // the launch plan: pinned, or the tuned choice for this configuration, or decided below
+    const int cap_single = std::min(nmax, RBFX_MAX_N);
+    const bool single_ok = rbfx::sym_fits(cap_single, sh.n_cols);
+    const TuneKey pkey{a->dim,        sh.n_cols, a->precision, kPlanKey, sh.kmax, sh.m, sh.basis_kind,
+                       sh.hermite ? 1 : 0, sh.uniform ? 1 : 0};
+    int plan = g_force_plan;
+    if (plan == 0) {
+        auto it = ds.tuned.find(pkey);
+        auto jt = g_tune_saved.find(pkey);
+        if (it != ds.tuned.end()) plan = it->second;
+        else if (jt != g_tune_saved.end()) plan = jt->second;
+    }
+    if (plan == kPlanSingle && !single_ok) plan = kPlanBucketed;
+    auto launch_single = [&]() -> int {
+        // one launch over every stencil in processing order, each wavefront sized by
+        // its own stencil (the packed kernel's lane slots above the stencil stay idle)
+        for (int b = 0; b < 3; ++b) {
+            g_bucket_n[b] = -1;
+            g_bucket_kernel[b] = K_SYM;
+        }
+        g_stats[2] = kPlanSingle;
+        if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
+        HIPCHK(launch_kernel_id(K_SYM, P, sh, hb, 2, fp32, cap_single, a->n_eval, async, st));
+        return RBFX_OK;
+    };
+    if (plan == kPlanSingle) return launch_single();
     const size_t need = sizeof(int32_t) * 3 * (size_t)a->n_eval;
     if (!async) {
         HIPCHK(grow((void**)&ds.bucket_list, ds.cap_bucket, need));
@@ -559,13 +609,42 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Sh
     }
     const int rows_cap[3] = {32, 64, std::min(nmax, RBFX_MAX_N)};
     int kid[3];
+    float per_stencil[3];
     for (int b = 0; b < 3; ++b) {
         rbfx::BuildDev Q = P;
         Q.list = ds.bucket_list + (int64_t)b * a->n_eval;
         Q.list_len = ds.bucket_count + 3 + b;
-        const int rc =
-            pick_kernel(a, Q, sh, ds, hb, b, fp32, rows_cap[b], cnt[b], ds.bucket_count + 3 + b, mode, st, &kid[b]);
+        const int rc = pick_kernel(a, Q, sh, ds, hb, b, fp32, rows_cap[b], cnt[b], ds.bucket_count + 3 + b, mode, st,
+                                   &kid[b], &per_stencil[b]);
         if (rc != RBFX_OK) return rc;
+    }
+    // Plan choice, once per configuration, when every nonempty bucket was just timed:
+    // the single launch's kernel on each bucket's sample against the bucket's own pick,
+    // weighted by the bucket counts.
+    if (plan == 0 && !async && single_ok) {
+        bool all_timed = true;
+        for (int b = 0; b < 3; ++b)
+            if (cnt[b] > 0 && per_stencil[b] < 0.f) all_timed = false;
+        if (all_timed) {
+            double est_bucketed = 0.0, est_single = 0.0;
+            for (int b = 0; b < 3; ++b) {
+                if (cnt[b] <= 0) continue;
+                const int32_t sample = (int32_t)std::min<int64_t>(cnt[b], kTuneSample);
+                rbfx::BuildDev S = P;
+                S.list = ds.bucket_list + (int64_t)b * a->n_eval;
+                S.list_len = ds.bucket_count + 3 + b;
+                hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, st, ds.bucket_count + 3 + b, sample);
+                float ms = 0.f;
+                const int rc = time_sample(K_SYM, S, sh, ds, hb, 2, fp32, cap_single, sample, st, &ms);
+                if (rc != RBFX_OK) return rc;
+                est_bucketed += (double)cnt[b] * per_stencil[b];
+                est_single += (double)cnt[b] * ms / sample;
+            }
+            g_stats[0] = 0;  // (sample runs)
+            plan = est_single < est_bucketed ? kPlanSingle : kPlanBucketed;
+            ds.tuned[pkey] = plan;
+            if (plan == kPlanSingle) return launch_single();
+        }
     }
     if (fp32) hipLaunchKernelGGL(k_zero_i32, dim3(1), dim3(1), 0, st, ds.fail_count);
     for (int b = 0; b < 3; ++b) {
@@ -577,6 +656,7 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Sh
         Q.list_len = ds.bucket_count + b;
         HIPCHK(launch_kernel_id(kid[b], Q, sh, hb, b, fp32, rows_cap[b], cnt[b], async, st));
     }
+    g_stats[2] = kPlanBucketed;
     return RBFX_OK;
 }
 
@@ -989,6 +1069,15 @@ int rbfx_last_build_stats(int64_t stats[8])
     return RBFX_OK;
 }
 
+int rbfx_tune_plan(int32_t plan)
+{
+    if (plan < 0 || plan > kPlanSingle) return set_err(RBFX_EINVAL, "plan must be 0, 1 or 2 (got %d)", plan);
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_force_plan = plan;
+    g_err.clear();
+    return RBFX_OK;
+}
+
 // Text format, one configuration per line:
 //   dim n_cols precision bucket kmax m basis_kind hermite uniform kernel
 int rbfx_tune_save(const char* path)
@@ -1039,7 +1128,8 @@ int rbfx_tune_load(const char* path)
         int v[10];
         if (std::sscanf(line, "%d %d %d %d %d %d %d %d %d %d", &v[0], &v[1], &v[2], &v[3], &v[4], &v[5], &v[6], &v[7],
                         &v[8], &v[9]) != 10 ||
-            v[9] < 0 || v[9] >= kNumKernels || v[3] < 0 || v[3] > 2) {
+            v[3] < 0 || v[3] > kPlanKey || v[9] < 0 || v[9] >= kNumKernels ||
+            (v[3] == kPlanKey && v[9] != kPlanBucketed && v[9] != kPlanSingle)) {
             std::fclose(f);
             return set_err(RBFX_EINVAL, "rbfx_tune_load: %s line %d is not a tuning entry", path, lineno);
         }

@@ -23,6 +23,30 @@ def test_library_exports_declared_symbols(RBF):
     assert set(syms) <= set(RBF._lib.exported_symbols())
 
 
+def test_tune_plan_and_plan_entries_without_gpu(RBF, tmp_path):
+    """The launch-plan pin takes 0 (automatic), 1 (one launch per size bucket) or 2 (one
+    launch for all stencils) and refuses anything else; a tuning map may carry plan lines
+    (bucket field 3) naming plan 1 or 2 only. No GPU call is made."""
+    lib = C.CDLL(str(ROOT / "radialbasisfunctions.jl_amd" / "librbfx.so"))
+    lib.rbfx_tune_plan.argtypes = [C.c_int32]
+    lib.rbfx_tune_load.argtypes = [C.c_char_p]
+    lib.rbfx_tune_reset.restype = C.c_int
+    lib.rbfx_last_error.restype = C.c_char_p
+    for plan in (2, 1, 0):
+        assert lib.rbfx_tune_plan(plan) == 0
+    for plan in (-1, 3):
+        assert lib.rbfx_tune_plan(plan) != 0
+        assert b"plan" in lib.rbfx_last_error()
+    good = tmp_path / "plan.txt"
+    good.write_text("# dim n_cols precision bucket kmax m basis hermite uniform kernel\n2 1 0 3 60 10 3 1 0 2\n")
+    assert lib.rbfx_tune_load(str(good).encode()) == 0
+    for line in ("2 1 0 3 60 10 3 1 0 5\n", "2 1 0 4 60 10 3 1 0 2\n"):
+        bad = tmp_path / "bad.txt"
+        bad.write_text(line)
+        assert lib.rbfx_tune_load(str(bad).encode()) != 0
+    assert lib.rbfx_tune_reset() == 0
+
+
 def test_abi_version_without_gpu(RBF):
     lib = C.CDLL(str(ROOT / "radialbasisfunctions.jl_amd" / "librbfx.so"))
     lib.rbfx_abi_version.restype = C.c_int32

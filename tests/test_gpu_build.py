@@ -753,6 +753,90 @@ def test_graph_capture_uneven_build(RBF, gpu, precision):
         assert torch.equal(y[0, :, 0], y_ref), rep
 
 
+def _uneven_disk_build(RBF, gpu, precision="fp64"):
+    from scipy.spatial import cKDTree
+
+    rng = np.random.default_rng(19)
+    n = 60_000  # every size bucket above the autotuner's 4,096-stencil threshold
+    pts = rng.random((n, 2))
+    ks = rng.integers(12, 61, n)
+    _, full = cKDTree(pts).query(pts, 60)
+    adjl = [full[i, :ks[i]] for i in range(n)]
+    deg = rng.integers(2, 4, n).astype(np.int8)
+    return pts, lambda: RBF.laplacian(pts, basis=RBF.PHS(3, poly_deg=3), adjl=adjl, device=gpu,
+                                      precision=precision, poly_deg_map=deg)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_refined"])
+def test_single_launch_plan(RBF, gpu, precision):
+    """The one-launch plan of an uneven build (ragged k 12..60, degree 2 or 3: systems of
+    18..70 rows) runs the packed LDS kernel over every stencil in processing order, with
+    no classify pass. Its weights equal, bit for bit, the bucketed plan's with that
+    kernel pinned for every bucket (the same per-stencil arithmetic at any lane-slot
+    count), and its statuses account for every stencil; captured in a graph with its
+    apply, every replay reproduces the eager weights and applied values."""
+    import torch
+
+    pts, build = _uneven_disk_build(RBF, gpu, precision)
+    try:
+        for b in range(3):
+            RBF._lib.tune_bucket(b, 7)
+        RBF._lib.tune_plan(1)
+        ref = build()
+        assert RBF._lib.last_build_stats()[2] == 1
+        assert all(c > 0 for c in RBF._lib.last_build_buckets()[0])
+        for b in range(3):
+            RBF._lib.tune_bucket(b, -1)
+        RBF._lib.tune_plan(2)
+        op = build()
+        assert RBF._lib.last_build_stats()[2] == 2
+        assert RBF._lib.last_build_buckets()[1] == [7, 7, 7]
+        assert torch.equal(op.vals, ref.vals)
+        st = op.status_counts
+        assert st[RBF._abi.ST_OK] + st[RBF._abi.ST_FP64] == len(pts), st
+        w_eager = op.vals.clone()
+        f0 = np.sin(4 * pts[:, 0]) * np.cos(3 * pts[:, 1])
+        G = op.capture(f0)
+        for rep in range(3):
+            f = f0 * (1.0 + 0.5 * rep) + rep * pts[:, 0]
+            G.x.copy_(torch.as_tensor(f, device=gpu)[:, None])
+            op.vals.fill_(0.0)
+            torch.cuda.synchronize()
+            y = G.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(op.vals, w_eager), rep
+            y_ref = op(torch.as_tensor(f, device=gpu))
+            assert torch.equal(y[0, :, 0], y_ref), rep
+    finally:
+        for b in range(3):
+            RBF._lib.tune_bucket(b, -1)
+        RBF._lib.tune_plan(0)
+
+
+def test_launch_plan_autotuned(RBF, gpu):
+    """With the plan automatic, the first uneven build of a configuration whose buckets
+    all hold 4,096+ stencils times both plans and keeps the faster; a second build of
+    the same configuration reuses the choice with no sample launches, and the map saved
+    with tune_save carries it as a plan line."""
+    import tempfile
+
+    RBF._lib.tune_reset()
+    _, build = _uneven_disk_build(RBF, gpu)
+    try:
+        build()
+        plan = RBF._lib.last_build_stats()[2]
+        assert plan in (1, 2) and RBF._lib.last_build_stats()[1] > 0
+        build()
+        assert RBF._lib.last_build_stats()[1] == 0 and RBF._lib.last_build_stats()[2] == plan
+        with tempfile.TemporaryDirectory() as d:
+            path = f"{d}/map.txt"
+            RBF._lib.tune_save(path)
+            rows = [l.split() for l in open(path) if not l.startswith("#")]
+        assert [r for r in rows if r[3] == "3" and r[9] == str(plan)]
+    finally:
+        RBF._lib.tune_reset()
+
+
 # ------------------------------------------------ config-shaped cases (reduced N)
 def _raw_build(RBF, gpu, pts, adj, basis, n_cols, terms, precision=0):
     """One build through the C ABI with an arbitrary term table (fused RHS columns)."""
