@@ -607,7 +607,9 @@ static int solve_all(const rbfx_build_args* a, const rbfx::BuildDev& P, const Sh
         HIPCHK(hipMemcpyAsync(cnt, ds.bucket_count, sizeof(cnt), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
     }
-    const int rows_cap[3] = {32, 64, std::min(nmax, RBFX_MAX_N)};
+    // (the LDS kernels size their tables by rows_cap: the bucket's bound or the largest
+    // stencil of the build, whichever is smaller)
+    const int rows_cap[3] = {std::min(32, nmax), std::min(64, nmax), std::min(nmax, RBFX_MAX_N)};
     int kid[3];
     float per_stencil[3];
     for (int b = 0; b < 3; ++b) {

@@ -680,7 +680,10 @@ constexpr bool sym_rows_spread()
 }
 static_assert(sym_rows_spread(), "row starts spread over the LDS banks");
 
-template <int DIM, int RC, int TWO>
+// HERM: the build may carry boundary data. Launches without any (P.is_boundary null)
+// whose LDS is small enough use the instantiation without the Hermite paths, whose
+// smaller register allocation leaves room for more wavefronts per SIMD
+template <int DIM, int RC, int TWO, bool HERM>
 __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym(const BuildDev P, int rows_cap,
                                                                                  int tri)
 {
@@ -696,7 +699,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
     double* s_ba = reinterpret_cast<double*>(s_bn + rows_cap);
     double* s_bb = s_ba + rows_cap;
     int* s_bt = reinterpret_cast<int*>(s_bb + rows_cap);
-    const bool bnd_tables = P.is_boundary != nullptr;
+    const bool bnd_tables = HERM && P.is_boundary != nullptr;
     __shared__ int8_t s_blk[ROWS];
     __shared__ int s_mono[kMaxMono][3];
     __shared__ TermDev s_terms[RBFX_MAX_TERMS];
@@ -749,14 +752,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                 r2 += D[t][d] * D[t][d];
             }
             r2max = fmax(r2max, r2);
-            ib[t] = P.is_boundary && P.is_boundary[gidx[t]];
+            ib[t] = HERM && P.is_boundary && P.is_boundary[gidx[t]];
         }
     }
     // stencil classification (types.jl:208-223)
-    const bool hermite = P.is_boundary && (__ballot(ib[0] || ib[TWO - 1]) != 0ull);
+    const bool hermite = HERM && P.is_boundary && (__ballot(ib[0] || ib[TWO - 1]) != 0ull);
     // a Dirichlet eval node is an identity row whatever its neighbours are (the row
     // length count_nonzeros gives it, execution.jl:39-69)
-    if (P.is_boundary && e < P.n_data && P.is_boundary[e]) {
+    if (HERM && P.is_boundary && e < P.n_data && P.is_boundary[e]) {
         const int bi = P.bnd_index[e];
         if (P.bc_alpha[bi] == 1.0 && P.bc_beta[bi] == 0.0) {
             // identity row (fill_dirichlet_entry!, execution.jl:297-303)
@@ -1339,11 +1342,17 @@ static hipError_t launch_sym_rc(const BuildDev& P, int rows_cap, int64_t n_work,
     Q.n_blocks = n_work;
     const int tri = sym_tri(rows_cap);
     const size_t lds = sym_lds(rows_cap, P.n_cols, DIM, P.is_boundary != nullptr);
-    if (rows_cap <= 64)
-        hipLaunchKernelGGL((k_build_sym<DIM, RC, 1>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
+    // the instantiation without Hermite paths (78..94 VGPRs against 120..130) where the
+    // LDS leaves room for more than the 12 wavefronts per CU of the other's registers
+    const bool lean = !P.is_boundary && rows_cap <= 64 && 160 * 1024 / (lds + kWideStaticLds) > 12;
+    if (lean)
+        hipLaunchKernelGGL((k_build_sym<DIM, RC, 1, false>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
+                           rows_cap, tri);
+    else if (rows_cap <= 64)
+        hipLaunchKernelGGL((k_build_sym<DIM, RC, 1, true>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
                            rows_cap, tri);
     else
-        hipLaunchKernelGGL((k_build_sym<DIM, RC, 2>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
+        hipLaunchKernelGGL((k_build_sym<DIM, RC, 2, true>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
                            rows_cap, tri);
     return hipGetLastError();
 }

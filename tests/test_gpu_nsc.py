@@ -214,7 +214,10 @@ def test_tuned_choice_never_given_to_ragged_build(RBF, O, gpu, tmp_path):
             RBF._lib.tune_load(path)
         op = RBF.laplacian(pv, basis=RBF.PHS(3, poly_deg=2), adjl=adjl)
         nb, kern = RBF._lib.last_build_buckets()
-        assert nb[1] == n and kern[1] != K_NSC, (nb, kern)
+        if RBF._lib.last_build_stats()[2] == 2:  # one launch of the packed LDS kernel
+            assert kern == [7, 7, 7], kern
+        else:
+            assert nb[1] == n and kern[1] != K_NSC, (nb, kern)
         W = op.weights()
         sel = rng.choice(n, 200, replace=False)
         aptr = np.concatenate([[0], np.cumsum([len(adjl[i]) for i in sel])])
