@@ -698,15 +698,21 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
     double(*s_bn)[DIM] = reinterpret_cast<double(*)[DIM]>(s_R + rows_cap * (RCP > DIM ? RCP : DIM));
     double* s_ba = reinterpret_cast<double*>(s_bn + rows_cap);
     double* s_bb = s_ba + rows_cap;
-    int* s_bt = reinterpret_cast<int*>(s_bb + rows_cap);
+    int8_t* s_bt = reinterpret_cast<int8_t*>(s_bb + rows_cap);
     const bool bnd_tables = HERM && P.is_boundary != nullptr;
+    // (byte-sized tables: the static LDS counts toward the wavefronts per CU)
     __shared__ int8_t s_blk[ROWS];
-    __shared__ int s_mono[kMaxMono][3];
+    __shared__ int8_t s_mono[kMaxMono][3];
     __shared__ TermDev s_terms[RBFX_MAX_TERMS];
 
     const int L = threadIdx.x;
     const int n_cols = P.n_cols, n_terms = P.n_terms;
-    for (int i = L; i < kMaxMono * 3; i += 64) (&s_mono[0][0])[i] = (&P.mono_e[0][0])[i];
+    for (int i = L; i < kMaxMono * 3; i += 64) (&s_mono[0][0])[i] = (int8_t)(&P.mono_e[0][0])[i];
+    auto mono_exps = [&](int a, int(&ex)[3]) {
+        ex[0] = s_mono[a][0];
+        ex[1] = s_mono[a][1];
+        ex[2] = s_mono[a][2];
+    };
     if (L < RBFX_MAX_TERMS) s_terms[L] = P.terms[L];
 
     // one stencil per workgroup: every exit below is uniform over the workgroup
@@ -797,7 +803,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                 eq = eq && (D[t][d] == 0.0);
             }
             if (bnd_tables) {
-                s_bt[q] = bt;
+                s_bt[q] = (int8_t)bt;
                 s_ba[q] = ba;
                 s_bb[q] = bb / h;  // beta acts through a first derivative
             }
@@ -865,8 +871,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             }
             // the point's monomial values: column j of monomial row a (position kk + a)
             for (int a = 0; a < M; ++a) {
-                double v = mono_op<DIM>(s_mono[a], RBFX_OP_VALUE, 0, 0, xi);
-                if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(s_mono[a], bn, xi);
+                int ea[3];
+                mono_exps(a, ea);
+                double v = mono_op<DIM>(ea, RBFX_OP_VALUE, 0, 0, xi);
+                if (hermite && bt == 2) v = ba * v + bb * mono_normal<DIM>(ea, bn, xi);
                 s_T[rb_of(kk + a) + j] = v;
             }
         }
@@ -944,7 +952,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             double z[DIM];
 #pragma unroll
             for (int d = 0; d < DIM; ++d) z[d] = 0.0;
-            const int(&ea)[3] = s_mono[j - kk];
+            int ea[3];
+            mono_exps(j - kk, ea);
             if (hermite && s_bt[eval_local] == 2) {
                 // the eval node's boundary operator applied to P (assembly.jl:452-467)
                 const double eav = s_ba[eval_local], ebv = s_bb[eval_local];
@@ -1320,13 +1329,13 @@ hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n
 
 // dynamic LDS of k_build_sym: the packed triangle, the right-hand sides (an area that
 // also holds the scaled points), and for builds with boundary data the normals, two
-// boundary coefficients and a boundary type per row
+// boundary coefficients and a one-byte boundary type per row
 static int sym_tri(int rows_cap) { return kSymRbHost.rb[rows_cap]; }
 static size_t sym_lds(int rows_cap, int n_cols, int dim, bool bnd)
 {
     const int rcp = rc_class(n_cols) | 1;
     return ((size_t)sym_tri(rows_cap) + (size_t)rows_cap * (size_t)std::max(rcp, dim)) * 8 +
-           (bnd ? (size_t)rows_cap * (dim + 2) * 8 + (size_t)rows_cap * 4 : 0);
+           (bnd ? (size_t)rows_cap * (dim + 2) * 8 + (size_t)rows_cap : 0);
 }
 
 bool sym_fits(int rows_cap, int n_cols)
