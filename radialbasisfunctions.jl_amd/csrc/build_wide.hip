@@ -689,9 +689,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
 {
     constexpr int ROWS = 64 * TWO;
     constexpr int RCP = RC | 1;  // right-hand-side row stride (odd)
+    static_assert(HERM || TWO == 1, "the instantiation without Hermite paths serves up to 64 rows");
     // the packed triangle (tri doubles), then the right-hand sides (rows_cap x RCP), whose
-    // area holds the scaled points during the assembly; for builds with boundary data
-    // the normals, boundary coefficients and types follow
+    // area holds the scaled points during the assembly (HERM; without Hermite paths each
+    // lane keeps its own point and the others are read from their lanes); for builds with
+    // boundary data the normals, boundary coefficients and types follow
     extern __shared__ __attribute__((aligned(16))) double s_T[];
     double* s_R = s_T + tri;
     double(*s_xi)[DIM] = reinterpret_cast<double(*)[DIM]>(s_R);
@@ -700,20 +702,17 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
     double* s_bb = s_ba + rows_cap;
     int8_t* s_bt = reinterpret_cast<int8_t*>(s_bb + rows_cap);
     const bool bnd_tables = HERM && P.is_boundary != nullptr;
-    // (byte-sized tables: the static LDS counts toward the wavefronts per CU)
+    // (the static LDS counts toward the wavefronts per CU: the block list is bytes, and
+    // the monomial exponents and operator terms are read from the kernel arguments)
     __shared__ int8_t s_blk[ROWS];
-    __shared__ int8_t s_mono[kMaxMono][3];
-    __shared__ TermDev s_terms[RBFX_MAX_TERMS];
 
     const int L = threadIdx.x;
     const int n_cols = P.n_cols, n_terms = P.n_terms;
-    for (int i = L; i < kMaxMono * 3; i += 64) (&s_mono[0][0])[i] = (int8_t)(&P.mono_e[0][0])[i];
     auto mono_exps = [&](int a, int(&ex)[3]) {
-        ex[0] = s_mono[a][0];
-        ex[1] = s_mono[a][1];
-        ex[2] = s_mono[a][2];
+        ex[0] = P.mono_e[a][0];
+        ex[1] = P.mono_e[a][1];
+        ex[2] = P.mono_e[a][2];
     };
-    if (L < RBFX_MAX_TERMS) s_terms[L] = P.terms[L];
 
     // one stencil per workgroup: every exit below is uniform over the workgroup
     const int64_t lb = xcd_contiguous_block(P.n_blocks);
@@ -781,6 +780,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
     if (!(h > 0.0)) h = 1.0;
     const double hinv = 1.0 / h;
     unsigned long long at_eval[TWO];
+    double xo[DIM];  // this lane's scaled point (read by the other lanes without HERM)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) xo[d] = 0.0;
 #pragma unroll
     for (int t = 0; t < TWO; ++t) {
         const int q = L + 64 * t;
@@ -799,7 +801,9 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             eq = hermite;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                s_xi[q][d] = D[t][d] / h;
+                const double xs = D[t][d] / h;
+                if (HERM) s_xi[q][d] = xs;
+                else xo[d] = xs;
                 eq = eq && (D[t][d] == 0.0);
             }
             if (bnd_tables) {
@@ -861,7 +865,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
         if (!is_poly) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                xi[d] = s_xi[j][d];
+                xi[d] = HERM ? s_xi[j][d] : xo[d];
                 bn[d] = hermite ? s_bn[j][d] : 0.0;
             }
             if (hermite) {
@@ -888,7 +892,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                     double q2 = 0.0;
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        const double t_ = xi[d] - s_xi[jj][d];
+                        const double t_ = xi[d] - (HERM ? s_xi[jj][d] : rdlane(xo[d], jj));
                         q2 += t_ * t_;
                     }
                     const double v = phi_r2_k<decltype(KC)::value>(B.eps, q2);
@@ -940,7 +944,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                 q2 += De[d] * De[d];
             }
             for (int tt = 0; tt < n_terms; ++tt) {
-                const TermDev tm = s_terms[tt];
+                const TermDev tm = P.terms[tt];
                 double v = phi_term<DIM>(B, tm.kind, tm.d1, tm.d2, De, q2);
                 if (hermite && bt == 2) v = ba * v + bb * phi_term_normal<DIM>(B, tm.kind, tm.d1, De, q2, bn);
                 const double sc = term_scale(tm, hinv);
@@ -965,7 +969,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                 for (int c = 0; c < RC; ++c) bcol[t][c] = v;
             } else {
                 for (int tt = 0; tt < n_terms; ++tt) {
-                    const TermDev tm = s_terms[tt];
+                    const TermDev tm = P.terms[tt];
                     const double v = term_scale(tm, hinv) * mono_op<DIM>(ea, tm.kind, tm.d1, tm.d2, z);
 #pragma unroll
                     for (int c = 0; c < RC; ++c)
@@ -1331,9 +1335,10 @@ hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n
 // also holds the scaled points), and for builds with boundary data the normals, two
 // boundary coefficients and a one-byte boundary type per row
 static int sym_tri(int rows_cap) { return kSymRbHost.rb[rows_cap]; }
-static size_t sym_lds(int rows_cap, int n_cols, int dim, bool bnd)
+static size_t sym_lds(int rows_cap, int n_cols, int dim, bool bnd, bool lean)
 {
     const int rcp = rc_class(n_cols) | 1;
+    if (lean) return ((size_t)sym_tri(rows_cap) + (size_t)rows_cap * (size_t)rcp) * 8;  // (no point table)
     return ((size_t)sym_tri(rows_cap) + (size_t)rows_cap * (size_t)std::max(rcp, dim)) * 8 +
            (bnd ? (size_t)rows_cap * (dim + 2) * 8 + (size_t)rows_cap : 0);
 }
@@ -1341,7 +1346,7 @@ static size_t sym_lds(int rows_cap, int n_cols, int dim, bool bnd)
 bool sym_fits(int rows_cap, int n_cols)
 {
     return rows_cap >= 1 && rows_cap <= RBFX_MAX_N &&
-           (int64_t)sym_lds(rows_cap, n_cols, 3, true) + kWideStaticLds <= 65536;
+           (int64_t)sym_lds(rows_cap, n_cols, 3, true, false) + kWideStaticLds <= 65536;
 }
 
 template <int DIM, int RC>
@@ -1350,10 +1355,12 @@ static hipError_t launch_sym_rc(const BuildDev& P, int rows_cap, int64_t n_work,
     BuildDev Q = P;
     Q.n_blocks = n_work;
     const int tri = sym_tri(rows_cap);
-    const size_t lds = sym_lds(rows_cap, P.n_cols, DIM, P.is_boundary != nullptr);
-    // the instantiation without Hermite paths (78..94 VGPRs against 120..130) where the
-    // LDS leaves room for more than the 12 wavefronts per CU of the other's registers
-    const bool lean = !P.is_boundary && rows_cap <= 64 && 160 * 1024 / (lds + kWideStaticLds) > 12;
+    // the instantiation without Hermite paths (78..94 VGPRs against 120..130, and no point
+    // table in LDS) where its LDS leaves room for more than the 12 wavefronts per CU of
+    // the other's registers
+    const size_t lean_lds = sym_lds(rows_cap, P.n_cols, DIM, false, true);
+    const bool lean = !P.is_boundary && rows_cap <= 64 && 160 * 1024 / (lean_lds + kWideStaticLds) > 12;
+    const size_t lds = lean ? lean_lds : sym_lds(rows_cap, P.n_cols, DIM, P.is_boundary != nullptr, false);
     if (lean)
         hipLaunchKernelGGL((k_build_sym<DIM, RC, 1, false>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
                            rows_cap, tri);
