@@ -1236,6 +1236,24 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             if (two) idet = 1.0 / (dp * dq - bq * bq);
             else dinv = 1.0 / dp;
             const int pl = pp & 63, pt = pp >> 6, ql = pq & 63, qt = pq >> 6;
+            // (the 1x1 and 2x2 solves are separate paths: a 1x1 pivot reads one right-hand
+            // side and updates with one column)
+            if (!two) {
+#pragma unroll
+                for (int c = 0; c < RC; ++c) {
+                    if (c >= n_cols) continue;
+                    const double rp0 = y[0][c] - acc[0][c];
+                    const double rp1 = TWO == 2 ? y[TWO - 1][c] - acc[TWO - 1][c] : 0.0;
+                    const double x0 = (pt ? rdlane(rp1, pl) : rdlane(rp0, pl)) * dinv;
+#pragma unroll
+                    for (int t = 0; t < TWO; ++t) {
+                        const int j = L + 64 * t;
+                        if (j == pp) x[t][c] = x0;
+                        if (j > pp && j < n) acc[t][c] += arp[t] * x0;
+                    }
+                }
+                continue;
+            }
 #pragma unroll
             for (int c = 0; c < RC; ++c) {
                 if (c >= n_cols) continue;
@@ -1243,14 +1261,14 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                 const double rp1 = TWO == 2 ? y[TWO - 1][c] - acc[TWO - 1][c] : 0.0;
                 const double yp = pt ? rdlane(rp1, pl) : rdlane(rp0, pl);
                 const double yq = qt ? rdlane(rp1, ql) : rdlane(rp0, ql);
-                const double x0 = two ? (yp * dq - yq * bq) * idet : yp * dinv;
-                const double x1 = two ? (yq * dp - yp * bq) * idet : 0.0;
+                const double x0 = (yp * dq - yq * bq) * idet;
+                const double x1 = (yq * dp - yp * bq) * idet;
 #pragma unroll
                 for (int t = 0; t < TWO; ++t) {
                     const int j = L + 64 * t;
                     if (j == pp) x[t][c] = x0;
-                    if (two && j == pq) x[t][c] = x1;
-                    if (j > pp && j < n) acc[t][c] += two ? arp[t] * x0 + arq[t] * x1 : arp[t] * x0;
+                    if (j == pq) x[t][c] = x1;
+                    if (j > pp && j < n) acc[t][c] += arp[t] * x0 + arq[t] * x1;
                 }
             }
         }
