@@ -116,3 +116,26 @@ def test_sym_matches_full_storage_kernel(RBF, gpu):
             ws.append(op.vals.cpu().numpy()[0].reshape(-1, k))
         d = np.abs(ws[0] - ws[1]).max(1) / np.abs(ws[0]).max(1)
         assert np.median(d) <= 1e-10 and np.quantile(d, 0.99) <= 1e-6, (k, np.median(d), np.quantile(d, 0.99), d.max())
+
+
+@pytest.mark.parametrize("dim,k,deg,which", [(2, 60, 3, "grad"), (3, 50, 3, "lap"), (3, 60, 2, "grad")])
+def test_sym_two_slots_multi_column(RBF, O, gpu, dim, k, deg, which):
+    """Systems of 70 rows (the second lane slot) with one or several right-hand sides, in
+    2-D and 3-D: k_build_sym<2,4,2>, <3,1,2> and <3,4,2>, against the extended-precision
+    oracle per stencil."""
+    rng = np.random.default_rng(60 + dim * k + deg)
+    pts = rng.random((1500, dim))
+    b = RBF.PHS(5, poly_deg=deg) if dim == 3 else RBF.PHS(3, poly_deg=deg)
+    if which == "grad":
+        L, nc = RBF.Jacobian(dim), dim
+        terms = [(c, O.abi.OP_D1, c, 0, 1.0) for c in range(dim)]
+    else:
+        L, nc = RBF.Laplacian(), 1
+        terms = [(0, O.abi.OP_LAP, 0, 0, 1.0)]
+    with pinned_sym(RBF):
+        op = RBF.RadialBasisOperator(L, pts, basis=b, k=k)
+        _ran_sym(RBF)
+        assert RBF._lib.last_build_buckets()[0][2] == len(pts)  # all in the 65..80-row bucket
+    adj = op.adj_idx.cpu().numpy()
+    B.check_against_truth(O, op.vals.cpu().numpy(), pts, pts, adj, O.basis_struct(b.kind, b.poly_deg),
+                          B._op(O, nc, terms), k)
