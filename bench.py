@@ -362,7 +362,23 @@ def config_c3(R, dev):
             # stencils without boundary points (degree 2): those with one carry Neumann rows
             # in place of plain ones, a different constraint
             out["poly_residual_interior"] = poly_residual(op, d, d, 2, keep=~touch)
+            w_auto = op.vals.clone()
         del op
+    # the same FP64 build as one launch of the packed LDS kernel over every stencil (the
+    # plan the autotuner weighs against the bucketed launches above)
+    R._lib.tune_plan(2)
+    try:
+        op = R.RadialBasisOperator(Lop, d, basis=R.PHS(3, poly_deg=3), adjl=(ptr, idx), hermite=herm,
+                                   precision="fp64", poly_deg_map=deg)
+        ms = _build_ms(R, op)
+        ok = bool(torch.isfinite(op.vals).all())
+        rel = float(((op.vals - w_auto).abs().max(dim=1).values).max() / w_auto.abs().max())
+        out["one_launch"] = {"build_ms": ms, "stencils_per_s": n / (ms * 1e-3),
+                             "launch_plan": "one launch for all stencils" if R._lib.last_build_stats()[2] == 2 else "-",
+                             "all_finite": ok, "max_abs_diff_vs_bucketed_over_max_weight": rel}
+        del op
+    finally:
+        R._lib.tune_plan(0)
     return out
 
 
