@@ -372,10 +372,19 @@ def config_c3(R, dev):
                                    precision="fp64", poly_deg_map=deg)
         ms = _build_ms(R, op)
         ok = bool(torch.isfinite(op.vals).all())
-        rel = float(((op.vals - w_auto).abs().max(dim=1).values).max() / w_auto.abs().max())
+        # per stencil: max |w_one - w_bucketed| / max |w_bucketed| (the <= 32-row stencils
+        # run on another kernel in the bucketed plan, so near-singular ones differ by their
+        # conditioning)
+        starts = ptr[:-1].cpu().numpy()
+        dd = np.maximum.reduceat((op.vals - w_auto).abs()[0].cpu().numpy(), starts)
+        ww = np.maximum.reduceat(w_auto.abs()[0].cpu().numpy(), starts)
+        rel = dd / np.maximum(ww, 1e-300)
         out["one_launch"] = {"build_ms": ms, "stencils_per_s": n / (ms * 1e-3),
                              "launch_plan": "one launch for all stencils" if R._lib.last_build_stats()[2] == 2 else "-",
-                             "all_finite": ok, "max_abs_diff_vs_bucketed_over_max_weight": rel}
+                             "all_finite": ok,
+                             "weights_vs_bucketed_rel_per_stencil": {
+                                 "median": float(np.median(rel)), "p99": float(np.quantile(rel, 0.99)),
+                                 "max": float(rel.max()), "within_1e-8": float((rel <= 1e-8).mean())}}
         del op
     finally:
         R._lib.tune_plan(0)
