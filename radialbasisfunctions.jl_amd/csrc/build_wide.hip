@@ -690,18 +690,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
     constexpr int ROWS = 64 * TWO;
     constexpr int RCP = RC | 1;  // right-hand-side row stride (odd)
     static_assert(HERM || TWO == 1, "the instantiation without Hermite paths serves up to 64 rows");
-    // the packed triangle (tri doubles), then the right-hand sides (rows_cap x RCP), whose
-    // area holds the scaled points during the assembly (HERM; without Hermite paths each
-    // lane keeps its own point and the others are read from their lanes); for builds with
-    // boundary data the normals, boundary coefficients and types follow
+    // the packed triangle (tri doubles), then the right-hand sides (rows_cap x RCP); the
+    // stencil points and their boundary data stay in the registers of the lanes that
+    // loaded them and are read by the others with lane reads
     extern __shared__ __attribute__((aligned(16))) double s_T[];
     double* s_R = s_T + tri;
-    double(*s_xi)[DIM] = reinterpret_cast<double(*)[DIM]>(s_R);
-    double(*s_bn)[DIM] = reinterpret_cast<double(*)[DIM]>(s_R + rows_cap * (RCP > DIM ? RCP : DIM));
-    double* s_ba = reinterpret_cast<double*>(s_bn + rows_cap);
-    double* s_bb = s_ba + rows_cap;
-    int8_t* s_bt = reinterpret_cast<int8_t*>(s_bb + rows_cap);
-    const bool bnd_tables = HERM && P.is_boundary != nullptr;
     // (the static LDS counts toward the wavefronts per CU: the block list is bytes, and
     // the monomial exponents and operator terms are read from the kernel arguments)
     __shared__ int8_t s_blk[ROWS];
@@ -780,40 +773,64 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
     if (!(h > 0.0)) h = 1.0;
     const double hinv = 1.0 / h;
     unsigned long long at_eval[TWO];
-    double xo[DIM];  // this lane's scaled point (read by the other lanes without HERM)
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) xo[d] = 0.0;
+    // this lane's points q = L + 64 t: the scaled coordinates and, for stencils with
+    // boundary points, the boundary type (1 Dirichlet, 2 Neumann/Robin), the coefficients
+    // (beta scaled by 1/h: it acts through a first derivative) and the normal
+    double xo[TWO][DIM], bno[TWO][DIM], bao[TWO], bbo[TWO];
+    int bto[TWO];
 #pragma unroll
     for (int t = 0; t < TWO; ++t) {
         const int q = L + 64 * t;
         bool eq = false;
+        bao[t] = 0.0;
+        bbo[t] = 0.0;
+        bto[t] = 0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+            xo[t][d] = 0.0;
+            bno[t][d] = 0.0;
+        }
         if (q < kk) {
-            int bt = 0;
-            double ba = 0.0, bb = 0.0;
             if (hermite && ib[t]) {
                 const int bi = P.bnd_index[gidx[t]];
-                ba = P.bc_alpha[bi];
-                bb = P.bc_beta[bi];
+                const double ba = P.bc_alpha[bi], bb = P.bc_beta[bi];
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) s_bn[q][d] = P.normals[(int64_t)bi * DIM + d];
-                bt = (ba == 1.0 && bb == 0.0) ? 1 : 2;
+                for (int d = 0; d < DIM; ++d) bno[t][d] = P.normals[(int64_t)bi * DIM + d];
+                bto[t] = (ba == 1.0 && bb == 0.0) ? 1 : 2;
+                bao[t] = ba;
+                bbo[t] = bb / h;
             }
             eq = hermite;
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                const double xs = D[t][d] / h;
-                if (HERM) s_xi[q][d] = xs;
-                else xo[d] = xs;
+                xo[t][d] = D[t][d] / h;
                 eq = eq && (D[t][d] == 0.0);
-            }
-            if (bnd_tables) {
-                s_bt[q] = (int8_t)bt;
-                s_ba[q] = ba;
-                s_bb[q] = bb / h;  // beta acts through a first derivative
             }
         }
         at_eval[t] = __ballot(eq);
     }
+    // point q's data (q uniform), from the lane that loaded it
+    auto pq_sel = [&](int q) -> int { return __builtin_amdgcn_readfirstlane(q); };
+    auto px = [&](int q, int d) -> double {
+        const int c = pq_sel(q);
+        return (TWO == 2 && c >= 64) ? rdlane(xo[TWO - 1][d], c & 63) : rdlane(xo[0][d], c & 63);
+    };
+    auto pbn = [&](int q, int d) -> double {
+        const int c = pq_sel(q);
+        return (TWO == 2 && c >= 64) ? rdlane(bno[TWO - 1][d], c & 63) : rdlane(bno[0][d], c & 63);
+    };
+    auto pba = [&](int q) -> double {
+        const int c = pq_sel(q);
+        return (TWO == 2 && c >= 64) ? rdlane(bao[TWO - 1], c & 63) : rdlane(bao[0], c & 63);
+    };
+    auto pbb = [&](int q) -> double {
+        const int c = pq_sel(q);
+        return (TWO == 2 && c >= 64) ? rdlane(bbo[TWO - 1], c & 63) : rdlane(bbo[0], c & 63);
+    };
+    auto pbt = [&](int q) -> int {
+        const int c = pq_sel(q);
+        return (TWO == 2 && c >= 64) ? rdlane(bto[TWO - 1], c & 63) : rdlane(bto[0], c & 63);
+    };
     int eval_local = 0;
     if (hermite) {
         const unsigned long long ae1 = TWO == 2 ? at_eval[TWO - 1] : 0ull;
@@ -865,13 +882,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
         if (!is_poly) {
 #pragma unroll
             for (int d = 0; d < DIM; ++d) {
-                xi[d] = HERM ? s_xi[j][d] : xo[d];
-                bn[d] = hermite ? s_bn[j][d] : 0.0;
+                xi[d] = xo[t][d];
+                bn[d] = bno[t][d];
             }
             if (hermite) {
-                bt = s_bt[j];
-                ba = s_ba[j];
-                bb = s_bb[j];
+                bt = bto[t];
+                ba = bao[t];
+                bb = bbo[t];
             }
             // the point's monomial values: column j of monomial row a (position kk + a)
             for (int a = 0; a < M; ++a) {
@@ -892,7 +909,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                     double q2 = 0.0;
 #pragma unroll
                     for (int d = 0; d < DIM; ++d) {
-                        const double t_ = xi[d] - (HERM ? s_xi[jj][d] : rdlane(xo[d], jj));
+                        const double t_ = xi[d] - px(jj, d);
                         q2 += t_ * t_;
                     }
                     const double v = phi_r2_k<decltype(KC)::value>(B.eps, q2);
@@ -903,8 +920,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             for (int jj = 0; jj <= j; ++jj) {
                 double xj[DIM];
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) xj[d] = s_xi[jj][d];
-                const int tj = s_bt[jj];
+                for (int d = 0; d < DIM; ++d) xj[d] = px(jj, d);
+                const int tj = pbt(jj);
                 double Dj[DIM], q2 = 0.0;
 #pragma unroll
                 for (int d = 0; d < DIM; ++d) {
@@ -915,10 +932,10 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                 if (bt == 2 || tj == 2) {
                     double nj[DIM];
 #pragma unroll
-                    for (int d = 0; d < DIM; ++d) nj[d] = s_bn[jj][d];
+                    for (int d = 0; d < DIM; ++d) nj[d] = pbn(jj, d);
                     double Di, Djn, D2;
                     phi_grad_hess_dirs<DIM>(B, Dj, q2, bn, nj, Di, Djn, D2);
-                    const double aj = s_ba[jj], bj = s_bb[jj];
+                    const double aj = pba(jj), bj = pbb(jj);
                     if (bt != 2) v = aj * v - bj * Djn;
                     else if (tj != 2) v = ba * v + bb * Di;
                     else v = ba * aj * v - ba * bj * Djn + bb * aj * Di + bb * bj * D2;
@@ -934,8 +951,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
         case RBFX_IMQ: point_cols(std::integral_constant<int, RBFX_IMQ>{}); break;
         default: point_cols(std::integral_constant<int, RBFX_GAUSSIAN>{}); break;
         }
-        // right-hand sides (kept in registers: their LDS area holds the points until the
-        // assembly ends)
+        // right-hand sides (kept in registers until the whole assembly is done)
         if (!is_poly) {
             double De[DIM], q2 = 0.0;
 #pragma unroll
@@ -958,12 +974,12 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             for (int d = 0; d < DIM; ++d) z[d] = 0.0;
             int ea[3];
             mono_exps(j - kk, ea);
-            if (hermite && s_bt[eval_local] == 2) {
+            if (hermite && pbt(eval_local) == 2) {
                 // the eval node's boundary operator applied to P (assembly.jl:452-467)
-                const double eav = s_ba[eval_local], ebv = s_bb[eval_local];
+                const double eav = pba(eval_local), ebv = pbb(eval_local);
                 double en[DIM];
 #pragma unroll
-                for (int d = 0; d < DIM; ++d) en[d] = s_bn[eval_local][d];
+                for (int d = 0; d < DIM; ++d) en[d] = pbn(eval_local, d);
                 const double v = eav * mono_op<DIM>(ea, RBFX_OP_VALUE, 0, 0, z) + ebv * mono_normal<DIM>(ea, en, z);
 #pragma unroll
                 for (int c = 0; c < RC; ++c) bcol[t][c] = v;
@@ -1349,22 +1365,17 @@ hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n
     }
 }
 
-// dynamic LDS of k_build_sym: the packed triangle, the right-hand sides (an area that
-// also holds the scaled points), and for builds with boundary data the normals, two
-// boundary coefficients and a one-byte boundary type per row
+// dynamic LDS of k_build_sym: the packed triangle and the right-hand sides
 static int sym_tri(int rows_cap) { return kSymRbHost.rb[rows_cap]; }
-static size_t sym_lds(int rows_cap, int n_cols, int dim, bool bnd, bool lean)
+static size_t sym_lds(int rows_cap, int n_cols)
 {
-    const int rcp = rc_class(n_cols) | 1;
-    if (lean) return ((size_t)sym_tri(rows_cap) + (size_t)rows_cap * (size_t)rcp) * 8;  // (no point table)
-    return ((size_t)sym_tri(rows_cap) + (size_t)rows_cap * (size_t)std::max(rcp, dim)) * 8 +
-           (bnd ? (size_t)rows_cap * (dim + 2) * 8 + (size_t)rows_cap : 0);
+    return ((size_t)sym_tri(rows_cap) + (size_t)rows_cap * (size_t)(rc_class(n_cols) | 1)) * 8;
 }
 
 bool sym_fits(int rows_cap, int n_cols)
 {
     return rows_cap >= 1 && rows_cap <= RBFX_MAX_N &&
-           (int64_t)sym_lds(rows_cap, n_cols, 3, true, false) + kWideStaticLds <= 65536;
+           (int64_t)sym_lds(rows_cap, n_cols) + kWideStaticLds <= 65536;
 }
 
 template <int DIM, int RC>
@@ -1373,12 +1384,10 @@ static hipError_t launch_sym_rc(const BuildDev& P, int rows_cap, int64_t n_work,
     BuildDev Q = P;
     Q.n_blocks = n_work;
     const int tri = sym_tri(rows_cap);
-    // the instantiation without Hermite paths (78..94 VGPRs against 120..130, and no point
-    // table in LDS) where its LDS leaves room for more than the 12 wavefronts per CU of
-    // the other's registers
-    const size_t lean_lds = sym_lds(rows_cap, P.n_cols, DIM, false, true);
-    const bool lean = !P.is_boundary && rows_cap <= 64 && 160 * 1024 / (lean_lds + kWideStaticLds) > 12;
-    const size_t lds = lean ? lean_lds : sym_lds(rows_cap, P.n_cols, DIM, P.is_boundary != nullptr, false);
+    // the instantiation without Hermite paths (fewer VGPRs) where the LDS leaves room for
+    // more than the 12 wavefronts per CU of the other's registers
+    const size_t lds = sym_lds(rows_cap, P.n_cols);
+    const bool lean = !P.is_boundary && rows_cap <= 64 && 160 * 1024 / (lds + kWideStaticLds) > 12;
     if (lean)
         hipLaunchKernelGGL((k_build_sym<DIM, RC, 1, false>), dim3((unsigned)pad_to_xcds(n_work)), dim3(64), lds, s, Q,
                            rows_cap, tri);
