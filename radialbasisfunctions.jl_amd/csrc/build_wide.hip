@@ -688,13 +688,11 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                                                                                  int tri)
 {
     constexpr int ROWS = 64 * TWO;
-    constexpr int RCP = RC | 1;  // right-hand-side row stride (odd)
     static_assert(HERM || TWO == 1, "the instantiation without Hermite paths serves up to 64 rows");
-    // the packed triangle (tri doubles), then the right-hand sides (rows_cap x RCP); the
-    // stencil points and their boundary data stay in the registers of the lanes that
-    // loaded them and are read by the others with lane reads
+    // the packed triangle (tri doubles) is the whole dynamic LDS: the right-hand sides,
+    // the stencil points and their boundary data stay in the registers of the lanes that
+    // own them and are read by the others with lane reads
     extern __shared__ __attribute__((aligned(16))) double s_T[];
-    double* s_R = s_T + tri;
     // (the static LDS counts toward the wavefronts per CU: the block list is bytes, and
     // the monomial exponents and operator terms are read from the kernel arguments)
     __shared__ int8_t s_blk[ROWS];
@@ -861,7 +859,8 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
         return (TWO == 2 && cc >= 64) ? rdlane(rbj[TWO - 1], cc & 63) : rdlane(rbj[0], cc & 63);
     };
 
-    // ---- assembly (FP64) of the lower triangle and the right-hand sides
+    // ---- assembly (FP64) of the lower triangle and the right-hand sides (registers: lane
+    // L holds those of its positions)
     double bcol[TWO][RC];
 #pragma unroll
     for (int t = 0; t < TWO; ++t) {
@@ -995,15 +994,6 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
         }
     }
     __syncthreads();
-#pragma unroll
-    for (int t = 0; t < TWO; ++t) {
-        const int j = L + 64 * t;
-        if (j < n) {
-#pragma unroll
-            for (int c = 0; c < RC; ++c) s_R[j * RCP + c] = bcol[t][c];
-        }
-    }
-    __syncthreads();
 
     // ---- factorisation with rook pivoting and symmetric moves of the pivots
     const double alpha = 0.6403882032022076;  // (1 + sqrt(17)) / 8
@@ -1060,10 +1050,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             s_T[x0] = y1;
             s_T[x1] = y0;
         }
-        for (int c = L; c < RC; c += 64) {
-            const double y0 = s_R[lo * RCP + c], y1 = s_R[hi * RCP + c];
-            s_R[lo * RCP + c] = y1;
-            s_R[hi * RCP + c] = y0;
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            const double ylo = (TWO == 2 && lo >= 64) ? rdlane(bcol[TWO - 1][c], lo & 63) : rdlane(bcol[0][c], lo & 63);
+            const double yhi = (TWO == 2 && hi >= 64) ? rdlane(bcol[TWO - 1][c], hi & 63) : rdlane(bcol[0][c], hi & 63);
+#pragma unroll
+            for (int t = 0; t < TWO; ++t) {
+                const int k = L + 64 * t;
+                if (k == lo) bcol[t][c] = yhi;
+                else if (k == hi) bcol[t][c] = ylo;
+            }
         }
         const int olo = (TWO == 2 && lo >= 64) ? rdlane(orig[TWO - 1], lo & 63) : rdlane(orig[0], lo & 63);
         const int ohi = (TWO == 2 && hi >= 64) ? rdlane(orig[TWO - 1], hi & 63) : rdlane(orig[0], hi & 63);
@@ -1142,8 +1138,13 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
         const int Q0 = n_act + kstep - 1;
         const double* U0 = s_T + rb_of(Q0);
         const double* U1 = s_T + rb_of(n_act);
-        const double* V0 = s_R + Q0 * RCP;
-        const double* V1 = s_R + n_act * RCP;
+        // the pivot rows' right-hand sides, from their lanes
+        double v0[RC], v1[RC];
+#pragma unroll
+        for (int c = 0; c < RC; ++c) {
+            v0[c] = (TWO == 2 && Q0 >= 64) ? rdlane(bcol[TWO - 1][c], Q0 & 63) : rdlane(bcol[0][c], Q0 & 63);
+            v1[c] = (TWO == 2 && n_act >= 64) ? rdlane(bcol[TWO - 1][c], n_act & 63) : rdlane(bcol[0][c], n_act & 63);
+        }
         // The trailing update of the active rows j < n_act over their columns 0..j, eight
         // at a time (the chunk past the diagonal lands in the row's padding), and of their
         // right-hand sides; per element the FMAs run in the same order as in k_build_wide.
@@ -1162,7 +1163,6 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
             }
             const double ml0 = -l0, ml1 = -l1;
             double* Tj = s_T + rbj[t];
-            double* Rj = s_R + j * RCP;
             auto chunk1 = [&](auto W, double* Ar, const double* W0, int j0) __attribute__((always_inline)) {
                 constexpr int CW = decltype(W)::value;
                 double a[CW], w0[CW];
@@ -1188,15 +1188,16 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
                 for (int q = 0; q < CW; ++q) Ar[j0 + q] = fma(ml1, w1[q], fma(ml0, w0[q], a[q]));
             };
             using C8 = std::integral_constant<int, 8>;
-            using CR = std::integral_constant<int, RC>;
             if (kstep == 1) {
                 for (int j0 = 0; j0 < n_act; j0 += 8)
                     if (j0 <= j) chunk1(C8{}, Tj, U0, j0);
-                chunk1(CR{}, Rj, V0, 0);
+#pragma unroll
+                for (int c = 0; c < RC; ++c) bcol[t][c] = fma(ml0, v0[c], bcol[t][c]);
             } else {
                 for (int j0 = 0; j0 < n_act; j0 += 8)
                     if (j0 <= j) chunk2(C8{}, Tj, U0, U1, j0);
-                chunk2(CR{}, Rj, V0, V1, 0);
+#pragma unroll
+                for (int c = 0; c < RC; ++c) bcol[t][c] = fma(ml1, v1[c], fma(ml0, v0[c], bcol[t][c]));
             }
         }
         slot += kstep;
@@ -1222,7 +1223,7 @@ __global__ __attribute__((amdgpu_flat_work_group_size(64, 64))) void k_build_sym
         for (int t = 0; t < TWO; ++t) {
             const int j = L + 64 * t;
 #pragma unroll
-            for (int c = 0; c < RC; ++c) y[t][c] = j < n ? s_R[j * RCP + c] : 0.0;
+            for (int c = 0; c < RC; ++c) y[t][c] = j < n ? bcol[t][c] : 0.0;
             // slot sl of the block list sits in lane sl % 64, register sl / 64
             bk[t] = j < nslots ? (int)s_blk[j] : -1;
         }
@@ -1365,11 +1366,12 @@ hipError_t launch_build_wide(const BuildDev& P, int dim, int rows_cap, int64_t n
     }
 }
 
-// dynamic LDS of k_build_sym: the packed triangle and the right-hand sides
+// dynamic LDS of k_build_sym: the packed triangle
 static int sym_tri(int rows_cap) { return kSymRbHost.rb[rows_cap]; }
 static size_t sym_lds(int rows_cap, int n_cols)
 {
-    return ((size_t)sym_tri(rows_cap) + (size_t)rows_cap * (size_t)(rc_class(n_cols) | 1)) * 8;
+    (void)n_cols;  // (the right-hand sides live in registers)
+    return (size_t)sym_tri(rows_cap) * 8;
 }
 
 bool sym_fits(int rows_cap, int n_cols)
