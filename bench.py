@@ -531,6 +531,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=200_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--model", default=str(MODEL_PATH))
+    ap.add_argument("--model-out", default=str(ROOT / "bench_out"),
+                    help="folder for a network trained in this run (when --model does not exist)")
     ap.add_argument("--no-configs", action="store_true", help="skip the other configurations (C2, C3, C5)")
     ap.add_argument("--only", choices=["c2", "c3", "c3_literal", "c5"],
                     help="run one other configuration alone and print it")
@@ -584,8 +586,8 @@ def main():
         model, hist, sweep = L.train_default(device=dev, log=lambda s: print(s, file=sys.stderr, flush=True))
         model_info.update(source="trained in this run", train_s=time.time() - t0, final_loss=hist[-1][1],
                           calibration=sweep)
-        out = ROOT / "gpurun_out"
-        out.mkdir(exist_ok=True)
+        out = pathlib.Path(args.model_out)
+        out.mkdir(parents=True, exist_ok=True)
         model.save(out / pathlib.Path(args.model).name)
     model_info["accept_tol"] = model.accept_tol
 

@@ -379,7 +379,8 @@ int rbfx_last_build_stats(int64_t stats[8]);
  * configuration is timed (every nonempty bucket holding 4096+ stencils) by timing
  * kernel 7, sized for the largest stencil, on each bucket's sample against that bucket's
  * own choice; it is kept in the tuning map (bucket field 3, kernel field the plan).
- * No reference counterpart (the north_star's one launch for stencils of uneven size). */
+ * No reference counterpart (one launch for stencils of uneven size, which the reference
+ * builds one stencil at a time on the CPU). */
 int rbfx_tune_plan(int32_t plan);
 
 /* Save / load the autotuner's choices (no reference counterpart; the checkpoint of
